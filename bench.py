@@ -1,0 +1,272 @@
+#!/usr/bin/env python3
+"""Benchmark: decompressed GB/s and % of the HBM roofline on one MI355X.
+
+Workload (one "step"): the FFOR bit-width sweep — for uint32 at every bit
+width 0..32 and uint64 at every bit width 0..64, a column of 2^28 values
+(per-vector random base plus uniform offsets in [0, 2^w)) is decoded to HBM by
+the HIP kernels. All packed inputs (~87.5 GB) are resident in HBM before
+timing; the output buffer (2 GiB) is reused. value = decompressed bytes of the
+whole sweep / step time.
+
+Extra, untimed-by-the-driver fields report the other shapes: the 2^25-value
+uint32 width-13 column (decode and fused sum), the fused filter + sum sweep,
+and per-width roofline fractions.
+
+`roofline` is measured on the dominant kernel (the uint64 FFOR decode) with
+HIP events around each launch on the launch stream. `cpu_baseline` times the
+reference's own CPU decoder (compiled from its sources into oracle/_ref) on a
+bounded sample of the same sweep, single-threaded.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "msc-thesis-project_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--values-log2", type=int, default=28, help="values per (dtype, width) column")
+    ap.add_argument("--variant", type=int, default=0, help="0 = autotuned default")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extras", action="store_true", help="only the timed sweep")
+    ap.add_argument("--out", default=None, help="also write the JSON line here")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); "
+              f"got --gpus {args.gpus}", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from flsgpu import _capi as A
+    from flsgpu import device as D
+    from flsgpu import encode as E
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    A.check(A.lib().flsgpu_init(0))
+    stream = torch.cuda.current_stream()
+    nvals = 1 << args.values_log2
+    n_vecs = nvals // 1024
+    print(f"[bench] building sweep columns: 2^{args.values_log2} values per width", flush=True)
+
+    cols = []  # (width, bw, DeviceFFORColumn)
+    for width in (32, 64):
+        for bw in range(width + 1):
+            p, b, o, s = E.synthetic_ffor_column_device(n_vecs, bw, width, seed=width * 100 + bw,
+                                                        device=dev)
+            cols.append((width, bw, D.DeviceFFORColumn(nvals, p, b, o, s, width)))
+    out32 = torch.empty(nvals, dtype=torch.uint32, device=dev)
+    out64 = torch.empty(nvals, dtype=torch.uint64, device=dev)
+    torch.cuda.synchronize()
+    print(f"[bench] resident: {torch.cuda.memory_allocated() / 1e9:.1f} GB", flush=True)
+
+    def algo_bytes(width, bw):
+        nl = 1024 // width
+        read = n_vecs * (bw * nl * width // 8 + 1 + 8 + width // 8)  # packed + bw + offset + base
+        return read + nvals * width // 8
+
+    def run_step(events=None):
+        for i, (width, bw, col) in enumerate(cols):
+            out = out32 if width == 32 else out64
+            if events is not None:
+                events[i][0].record(stream)
+            D.decompress(col, out, variant=args.variant, stream=stream)
+            if events is not None:
+                events[i][1].record(stream)
+
+    for _ in range(args.warmup):
+        run_step()
+    torch.cuda.synchronize()
+
+    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in cols]
+    kernel_ms = [0.0] * len(cols)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_step(events)
+        torch.cuda.synchronize()
+        for i, (a, b) in enumerate(events):
+            kernel_ms[i] += a.elapsed_time(b)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    # the per-step synchronize to read events is part of the timed region
+    step_ms = (t1 - t0) * 1e3 / args.steps
+
+    dec_bytes = sum(nvals * w // 8 for w, _, _ in cols)
+    value = dec_bytes / (step_ms / 1e3) / 1e9
+
+    per_width = []
+    for i, (width, bw, _) in enumerate(cols):
+        ms = kernel_ms[i] / args.steps
+        ab = algo_bytes(width, bw)
+        per_width.append({"dtype": f"u{width}", "bw": bw, "us": round(ms * 1e3, 2),
+                          "decompressed_GBps": round(nvals * width / 8 / (ms / 1e3) / 1e9, 1),
+                          "hbm_frac": round(ab / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)})
+    u64 = [i for i, (w, _, _) in enumerate(cols) if w == 64]
+    avg_bytes = sum(algo_bytes(64, cols[i][1]) for i in u64) / len(u64)
+    avg_ms = sum(kernel_ms[i] for i in u64) / len(u64) / args.steps
+    achieved = avg_bytes / (avg_ms / 1e3) / 1e9
+    u32 = [i for i, (w, _, _) in enumerate(cols) if w == 32]
+    avg_bytes32 = sum(algo_bytes(32, cols[i][1]) for i in u32) / len(u32)
+    avg_ms32 = sum(kernel_ms[i] for i in u32) / len(u32) / args.steps
+
+    result = {
+        "metric": "decompressed GB/s + % of HBM roofline, plain vs batched/autotuned, 1 GPU",
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32+u64",
+        "data": "synthetic",
+        "config": {"workload": f"FFOR bit-width sweep: u32 w=0..32 and u64 w=0..64, "
+                               f"2^{args.values_log2} values per width, decode to HBM",
+                   "values_per_width": nvals, "variant": args.variant},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_ffor_decompress<uint64_t,...> (average over w=0..64)",
+                     "avg_launch_us": round(avg_ms * 1e3, 2),
+                     "avg_algorithmic_bytes": int(avg_bytes)},
+        "u32_sweep": {"avg_launch_us": round(avg_ms32 * 1e3, 2),
+                      "hbm_frac": round(avg_bytes32 / (avg_ms32 / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+    }
+
+    del cols
+    torch.cuda.empty_cache()
+    if not args.no_extras:
+        result["extras"] = extras(args, torch, A, D, E, dev, stream, out32)
+        result["per_width"] = per_width
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(args)
+    line = json.dumps(result)
+    print(line, flush=True)
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(line + "\n")
+
+
+def time_launches(torch, stream, fn, reps):
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    fn()
+    torch.cuda.synchronize()
+    a.record(stream)
+    for _ in range(reps):
+        fn()
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def extras(args, torch, A, D, E, dev, stream, out32):
+    """Other configurations (not the headline): C1 u32 w=13 column, fused
+    filter+sum sweep at 2^28 values."""
+    import numpy as np
+    res = {}
+    # 2^25 uint32 values 1e6 + U[0, 2^13), FOR base = vector min, width 13
+    rng = np.random.default_rng(2025)
+    vals = (1_000_000 + rng.integers(0, 1 << 13, 1 << 25)).astype(np.uint32)
+    p, b, o, s = E.ffor_encode_uniform(vals, 13)
+    cols = [D.DeviceFFORColumn.from_host(p, b, o, s, len(vals), dev) for _ in range(4)]
+    out = torch.empty(len(vals), dtype=torch.uint32, device=dev)
+    k = [0]
+
+    def dec():
+        D.decompress(cols[k[0] % 4], out, variant=args.variant, stream=stream)
+        k[0] += 1
+    ms = time_launches(torch, stream, dec, 50)
+    nbytes = len(vals) * 4
+    algo = p.nbytes + b.nbytes + o.nbytes + s.nbytes + nbytes
+    res["c1_u32_w13_2^25_decode"] = {"us": round(ms * 1e3, 2),
+                                      "decompressed_GBps": round(nbytes / ms / 1e6, 1),
+                                      "hbm_frac": round(algo / ms / 1e6 / HBM_PEAK_GBS, 4)}
+    r = torch.empty(2, dtype=torch.uint64, device=dev)
+
+    def fsum():
+        D.query_filter_sum(cols[k[0] % 4], A.PRED_RANGE, 0, 2 ** 32 - 1, r, stream=stream)
+        k[0] += 1
+    ms = time_launches(torch, stream, fsum, 50)
+    got_sum = int(r.cpu().numpy()[1])
+    res["c1_u32_w13_2^25_sum"] = {"us": round(ms * 1e3, 2),
+                                   "decompressed_equiv_GBps": round(nbytes / ms / 1e6, 1),
+                                   "hbm_frac": round((algo - nbytes) / ms / 1e6 / HBM_PEAK_GBS, 4),
+                                   "sum_ok": got_sum == int(vals.astype(np.uint64).sum())}
+    del cols
+    # fused filter (offset < 2^(w-1)) + sum over the sweep widths (2^28 values)
+    nvals = 1 << args.values_log2
+    n_vecs = nvals // 1024
+    fs = []
+    for width, bws in ((32, (1, 8, 13, 16, 24, 32)), (64, (1, 16, 32, 48, 64))):
+        for bw in bws:
+            pk, bb, oo, ss = E.synthetic_ffor_column_device(n_vecs, bw, width, seed=7 + bw, device=dev)
+            col = D.DeviceFFORColumn(nvals, pk, bb, oo, ss, width)
+            ms = time_launches(torch, stream, lambda: D.query_filter_sum(
+                col, A.PRED_OFFSET_LOWER_HALF, result=r, stream=stream), 10)
+            read = pk.numel() * pk.element_size() + n_vecs * (1 + 8 + width // 8)
+            fs.append({"dtype": f"u{width}", "bw": bw, "us": round(ms * 1e3, 1),
+                       "decompressed_equiv_GBps": round(nvals * width / 8 / ms / 1e6, 1),
+                       "hbm_frac": round(read / ms / 1e6 / HBM_PEAK_GBS, 4)})
+            del col, pk, bb, oo, ss
+    res["filter_sum_sweep"] = fs
+    torch.cuda.empty_cache()
+    return res
+
+
+def cpu_baseline(args):
+    """The reference's own CPU decoder (fls::unffor per vector, the loop of
+    src/engine/data.cuh:300-313), compiled from its sources into oracle/_ref,
+    on a bounded sample of the same sweep: 2^20 values per (dtype, width)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle_py as O
+    use_ref = O.ref() is not None
+    kind = "reference" if use_ref else "port"
+    rng = np.random.default_rng(5)
+    n_vecs = 1024
+    samples = []
+    for width in (32, 64):
+        ut = np.uint32 if width == 32 else np.uint64
+        for bw in range(width + 1):
+            nl = 1024 // width
+            packed = np.frombuffer(rng.bytes(n_vecs * bw * nl * width // 8 or 8), dtype=ut).copy()
+            col = O.FFORColumn(n_vecs * 1024, packed, np.full(n_vecs, bw, np.uint8),
+                               np.arange(n_vecs, dtype=np.uint64) * np.uint64(bw * nl),
+                               np.frombuffer(rng.bytes(n_vecs * width // 8), dtype=ut).copy())
+            samples.append(col)
+    dec_bytes = sum(c.n_values * c.packed.itemsize for c in samples)
+    passes = 0
+    t0 = time.perf_counter()
+    while True:
+        for c in samples:
+            O.ffor_column_decode(c, use_ref=use_ref)
+        passes += 1
+        if time.perf_counter() - t0 > 10.0:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": round(dec_bytes * passes / dt / 1e9, 3), "unit": "GB/s", "cores": 1,
+            "kind": kind,
+            "sample": f"same sweep (u32 w=0..32, u64 w=0..64) at 2^20 values per width, "
+                      f"{passes} passes, single thread, fls::unffor per vector"}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,235 @@
+/*
+ * flsgpu — MI355X (gfx950) FastLanes / ALP / ALP-RD decoder, C ABI.
+ *
+ * Plain C: device pointers, sizes and a hipStream_t; no C++ or torch types.
+ * Every entry point is asynchronous on `stream` (NULL = the null stream),
+ * returns FLSGPU_OK (0) or a negative status, and never calls exit().
+ * flsgpu_last_error() returns a message for the calling thread's last failure.
+ *
+ * The column descriptors mirror the reference's device column structs field
+ * by field (src/flsgpu/structs.cuh:33-84): a packed array of W-bit words,
+ * per-vector bit widths (uint8), per-vector word offsets (uint64) and, for
+ * FFOR, per-vector bases. A vector holds 1024 values; n_values must be
+ * n_vecs * 1024 (the reference's format has full vectors only,
+ * src/engine/data.cuh:206, src/alp/alp-bindings.cu:116-126).
+ *
+ * Unlike the reference (structs.cuh:104-109, 160-177), the kernels read no
+ * word beyond a vector's own bit_width * N_LANES words, so no over-read
+ * padding is needed after the packed array or the exception arrays.
+ *
+ * Reductions (query_any, filter_sum, compute, multi-column) write their final
+ * result to a caller-provided device pointer from the kernel itself; they use
+ * a small per-device scratch area owned by the library, so concurrent calls
+ * of these reduction entry points on different streams of one device race.
+ */
+#ifndef FLSGPU_H
+#define FLSGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t *flsgpu_stream_t; /* == hipStream_t */
+
+enum flsgpu_status {
+  FLSGPU_OK = 0,
+  FLSGPU_EINVAL = -1,       /* bad argument: null pointer, size, enum value */
+  FLSGPU_EUNSUPPORTED = -2, /* variant / patcher / type combination not provided */
+  FLSGPU_EHIP = -3,         /* a HIP runtime call failed */
+  FLSGPU_ENOMEM = -4
+};
+
+/* Reference enums::Unpacker values (src/engine/enums.cuh:21-41), accepted as
+ * the `unpacker` argument of the C++ shim; the C ABI takes a native variant. */
+enum flsgpu_variant {
+  FLSGPU_VARIANT_AUTO = 0,        /* autotuned table, else the default below */
+  FLSGPU_VARIANT_LANE = 1,        /* one thread per FastLanes lane, full lane in registers */
+  FLSGPU_VARIANT_LANE_NT = 2,     /* as LANE, non-temporal (streaming) stores */
+  FLSGPU_VARIANT_HALF_LANE = 3,   /* two threads per lane, each half of the slots */
+  FLSGPU_VARIANT_HALF_LANE_NT = 4,
+  FLSGPU_VARIANT_QUARTER_LANE = 5,/* four threads per lane */
+  FLSGPU_VARIANT_QUARTER_LANE_NT = 6,
+  FLSGPU_VARIANT_COUNT = 7
+};
+
+/* Reference enums::Patcher values (src/engine/enums.cuh:43-53). */
+enum flsgpu_patcher {
+  FLSGPU_PATCHER_NONE = 0,
+  FLSGPU_PATCHER_DUMMY = 1,
+  FLSGPU_PATCHER_STATELESS = 2,
+  FLSGPU_PATCHER_STATEFUL = 3,
+  FLSGPU_PATCHER_NAIVE = 4,
+  FLSGPU_PATCHER_NAIVE_BRANCHLESS = 5,
+  FLSGPU_PATCHER_PREFETCH_POSITION = 6,
+  FLSGPU_PATCHER_PREFETCH_ALL = 7,
+  FLSGPU_PATCHER_PREFETCH_ALL_BRANCHLESS = 8
+};
+
+/* device::BPColumn<T> (src/flsgpu/structs.cuh:33-41) */
+typedef struct flsgpu_bp_column {
+  uint64_t n_values;
+  uint64_t n_vecs;
+  const void *packed_array;        /* uint32_t or uint64_t words */
+  const uint8_t *bit_widths;       /* [n_vecs] */
+  const uint64_t *vector_offsets;  /* [n_vecs], in words */
+} flsgpu_bp_column;
+
+/* device::FFORColumn<T> (structs.cuh:43-48) */
+typedef struct flsgpu_ffor_column {
+  flsgpu_bp_column bp;
+  const void *bases;               /* [n_vecs] W-bit bases */
+} flsgpu_ffor_column;
+
+/* device::ALPColumn<T> / ALPExtendedColumn<T> (structs.cuh:50-84). The
+ * FACT/FRAC tables are compiled into the kernels instead of being uploaded
+ * per column. Exactly one of counts (standard layout) and offsets_counts
+ * (G-ALP lane-divided layout, (count << 10) | offset per lane) is used. */
+typedef struct flsgpu_alp_column {
+  flsgpu_ffor_column ffor;         /* unsigned words of the float's width */
+  const uint8_t *factor_indices;   /* [n_vecs] */
+  const uint8_t *fraction_indices; /* [n_vecs] */
+  uint64_t n_exceptions;
+  const uint64_t *exceptions_offsets; /* [n_vecs] */
+  const void *exceptions;          /* float / double */
+  const uint16_t *positions;       /* 0..1023 within the vector */
+  const uint16_t *counts;          /* [n_vecs]            (ALPColumn) */
+  const uint16_t *offsets_counts;  /* [n_vecs * N_LANES]  (ALPExtendedColumn) */
+} flsgpu_alp_column;
+
+/* ALP-RD column. The reference has no packed ALP-RD format (src/alp/rd.hpp
+ * decodes unpacked parts only); this one follows the upstream-ALP convention
+ * of rd.hpp:48-56: right parts bit-packed with FFOR base 0 at right_bit_width
+ * (vector v at word v * right_bit_width * N_LANES), left parts as dictionary
+ * indices bit-packed as uint16 at left_bit_width (vector v at
+ * v * left_bit_width * 64 words), exceptions = original left parts. */
+typedef struct flsgpu_alprd_column {
+  uint64_t n_values;
+  uint64_t n_vecs;
+  uint32_t right_bit_width;        /* W-16 .. W-1 */
+  uint32_t left_bit_width;         /* 1 .. 3 */
+  uint32_t dictionary_size;        /* 1 .. 8 */
+  uint16_t dictionary[8];
+  const void *right_packed;        /* uint32_t / uint64_t words */
+  const uint16_t *left_packed;
+  const uint64_t *exceptions_offsets; /* [n_vecs] */
+  const uint16_t *exceptions;      /* original left parts */
+  const uint16_t *positions;
+  const uint16_t *counts;          /* [n_vecs] */
+} flsgpu_alprd_column;
+
+/* Fused filter predicates for flsgpu_query_filter_sum_*. */
+enum flsgpu_predicate {
+  /* value in [lo, hi) */
+  FLSGPU_PRED_RANGE = 0,
+  /* FOR offset (value - vector base) below half of the vector's range:
+   * offset < 2^(bit_width-1); no row passes at bit_width 0. */
+  FLSGPU_PRED_OFFSET_LOWER_HALF = 1
+};
+
+/* ---- library ------------------------------------------------------------ */
+const char *flsgpu_last_error(void);
+/* Optional; the first call on a device initialises it lazily. */
+int flsgpu_init(int device);
+/* Loads the autotuner table (JSON written by flsgpu.autotune); NULL resets. */
+int flsgpu_set_tuning(uint32_t dtype_bits, uint32_t bit_width, int variant);
+int flsgpu_get_tuning(uint32_t dtype_bits, uint32_t bit_width);
+
+/* ---- device memory helpers (replace GPUArray / copy_to_device,
+ *      src/flsgpu/host-utils.cuh:42-111, structs.cuh:103-229) --------------- */
+int flsgpu_upload(void **d_ptr, const void *h_ptr, size_t bytes, size_t pad_bytes,
+                  flsgpu_stream_t stream);
+int flsgpu_download(void *h_ptr, const void *d_ptr, size_t bytes, flsgpu_stream_t stream);
+int flsgpu_free(void *d_ptr);
+
+/* ---- decompress: kernels::host::decompress_column (src/engine/kernels.cuh:94-112),
+ *      bindings::decompress_column (src/generated-bindings/kernel-bindings.cuh:11-16,37-175) */
+int flsgpu_decompress_bp_u32(const flsgpu_bp_column *col, uint32_t *out, int variant,
+                             flsgpu_stream_t stream);
+int flsgpu_decompress_bp_u64(const flsgpu_bp_column *col, uint64_t *out, int variant,
+                             flsgpu_stream_t stream);
+int flsgpu_decompress_ffor_u32(const flsgpu_ffor_column *col, uint32_t *out, int variant,
+                               flsgpu_stream_t stream);
+int flsgpu_decompress_ffor_u64(const flsgpu_ffor_column *col, uint64_t *out, int variant,
+                               flsgpu_stream_t stream);
+/* ALPColumn: patcher in {NONE, DUMMY, STATELESS, STATEFUL} (Dummy/None skip patching).
+ * ALPExtendedColumn: patcher in {NAIVE, NAIVE_BRANCHLESS, PREFETCH_POSITION,
+ * PREFETCH_ALL, PREFETCH_ALL_BRANCHLESS} (generate-kernel-bindings.py:355-357).
+ * All patchers produce identical output; they map to the native patch paths. */
+int flsgpu_decompress_alp_f32(const flsgpu_alp_column *col, float *out, int patcher, int variant,
+                              flsgpu_stream_t stream);
+int flsgpu_decompress_alp_f64(const flsgpu_alp_column *col, double *out, int patcher, int variant,
+                              flsgpu_stream_t stream);
+int flsgpu_decompress_alp_ext_f32(const flsgpu_alp_column *col, float *out, int patcher,
+                                  int variant, flsgpu_stream_t stream);
+int flsgpu_decompress_alp_ext_f64(const flsgpu_alp_column *col, double *out, int patcher,
+                                  int variant, flsgpu_stream_t stream);
+/* ALP-RD: no reference GPU counterpart (rd.hpp:146-172 is CPU-only, uncalled). */
+int flsgpu_decompress_alprd_f32(const flsgpu_alprd_column *col, float *out, int variant,
+                                flsgpu_stream_t stream);
+int flsgpu_decompress_alprd_f64(const flsgpu_alprd_column *col, double *out, int variant,
+                                flsgpu_stream_t stream);
+
+/* ---- query "any value == magic": kernels::host::query_column
+ *      (src/engine/kernels.cuh:114-133, device :34-54, MagicChecker
+ *      src/engine/device-utils.cuh:122-143). *d_found (uint8) is written with
+ *      0 or 1 by the kernel (the reference only writes on a hit). */
+int flsgpu_query_any_bp_u32(const flsgpu_bp_column *col, uint32_t magic, uint8_t *d_found,
+                            int variant, flsgpu_stream_t stream);
+int flsgpu_query_any_bp_u64(const flsgpu_bp_column *col, uint64_t magic, uint8_t *d_found,
+                            int variant, flsgpu_stream_t stream);
+int flsgpu_query_any_ffor_u32(const flsgpu_ffor_column *col, uint32_t magic, uint8_t *d_found,
+                              int variant, flsgpu_stream_t stream);
+int flsgpu_query_any_ffor_u64(const flsgpu_ffor_column *col, uint64_t magic, uint8_t *d_found,
+                              int variant, flsgpu_stream_t stream);
+int flsgpu_query_any_alp_f32(const flsgpu_alp_column *col, float magic, uint8_t *d_found,
+                             int patcher, int variant, flsgpu_stream_t stream);
+int flsgpu_query_any_alp_f64(const flsgpu_alp_column *col, double magic, uint8_t *d_found,
+                             int patcher, int variant, flsgpu_stream_t stream);
+int flsgpu_query_any_alp_ext_f32(const flsgpu_alp_column *col, float magic, uint8_t *d_found,
+                                 int patcher, int variant, flsgpu_stream_t stream);
+int flsgpu_query_any_alp_ext_f64(const flsgpu_alp_column *col, double magic, uint8_t *d_found,
+                                 int patcher, int variant, flsgpu_stream_t stream);
+
+/* ---- fused decode + filter + sum (no reference counterpart: the reference
+ *      query kernel only answers "any == magic"). d_result[0] = number of
+ *      rows passing, d_result[1] = sum (mod 2^64) of the passing values. */
+int flsgpu_query_filter_sum_ffor_u32(const flsgpu_ffor_column *col, int predicate, uint64_t lo,
+                                     uint64_t hi, uint64_t *d_result, int variant,
+                                     flsgpu_stream_t stream);
+int flsgpu_query_filter_sum_ffor_u64(const flsgpu_ffor_column *col, int predicate, uint64_t lo,
+                                     uint64_t hi, uint64_t *d_result, int variant,
+                                     flsgpu_stream_t stream);
+
+/* ---- synthetic ALU load: kernels::host::compute_column (kernels.cuh:135-153,
+ *      device :56-89): per value n_repetitions of *=3, <<=3, +=3, >>=3, &=0,
+ *      then "any == 1"; *d_result is written with 0 or 1. */
+int flsgpu_compute_ffor_u32(const flsgpu_ffor_column *col, uint32_t n_repetitions,
+                            uint8_t *d_result, int variant, flsgpu_stream_t stream);
+int flsgpu_compute_ffor_u64(const flsgpu_ffor_column *col, uint32_t n_repetitions,
+                            uint8_t *d_result, int variant, flsgpu_stream_t stream);
+
+/* ---- multi-column query: multi_column::query_multi_column (arity 1..10,
+ *      src/generated-bindings/multi-column-device-kernels.cuh:10-443): the
+ *      answer is 1 iff some (vector, lane) has, in every slot, all columns
+ *      equal to each other and to `value` ... see DESIGN for the exact
+ *      reading; columns must have equal n_vecs. n_cols in 1..10. */
+int flsgpu_query_multi_column_ffor_u32(const flsgpu_ffor_column *cols, uint32_t n_cols,
+                                       uint32_t value, uint8_t *d_found, int variant,
+                                       flsgpu_stream_t stream);
+int flsgpu_query_multi_column_ffor_u64(const flsgpu_ffor_column *cols, uint32_t n_cols,
+                                       uint64_t value, uint8_t *d_found, int variant,
+                                       flsgpu_stream_t stream);
+int flsgpu_query_multi_column_alp_f32(const flsgpu_alp_column *cols, uint32_t n_cols,
+                                      float value, uint8_t *d_found, int patcher, int variant,
+                                      flsgpu_stream_t stream);
+int flsgpu_query_multi_column_alp_f64(const flsgpu_alp_column *cols, uint32_t n_cols,
+                                      double value, uint8_t *d_found, int patcher, int variant,
+                                      flsgpu_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLSGPU_H */

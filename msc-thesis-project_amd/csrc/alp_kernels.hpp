@@ -1,0 +1,211 @@
+// ALP / G-ALP decode kernels for gfx950. Included only by alp_inst.hip.
+//
+// Value formula, in this exact order (src/flsgpu/alp.cuh:40-45, CPU
+// src/alp/falp.cpp, e.g. :95-130):
+//   out = (T)(INT)((UINT)(unpacked + base) * (UINT)FACT[factor_idx]) * FRAC[fraction_idx]
+// i.e. wrapping unsigned add and multiply, reinterpretation as signed,
+// round-to-nearest-even integer->float conversion, one IEEE multiply. No FMA
+// contraction (built with -ffp-contract=off), no division by 10^e.
+//
+// Exception patching (src/flsgpu/alp.cuh:48-461, CPU decoder.hpp:137-145):
+// every exception is written by the thread that owns its (lane, slot), after
+// that thread stored the decoded value at the same address, so the final value
+// is the exception by single-thread program order. With the G-ALP layout a
+// thread reads only its own lane's exceptions through the per-lane header;
+// with the standard layout it scans the vector's list and keeps its lane's.
+#pragma once
+
+#include "alp_impl.hpp"
+#include "kernels.hpp"
+
+namespace flsgpu_hip {
+
+template <typename T>
+__device__ __forceinline__ T alp_value(typename uint_of<T>::type v, typename uint_of<T>::type base,
+                                       typename uint_of<T>::type fact, T frac) {
+  using UT = typename uint_of<T>::type;
+  using ST = typename int_of<T>::type;
+  const ST s = ST(UT(UT(v + base) * fact));
+  return T(s) * frac;
+}
+
+template <typename T, bool NT>
+struct AlpDecompressPolicy {
+  using UT = typename uint_of<T>::type;
+  const UT *in;
+  T *out;
+  UT base;
+  UT fact;
+  T frac;
+  template <int BW, int J0, int NJ>
+  __device__ __forceinline__ void run() {
+    unpack_slots<UT, BW, J0, NJ>(in, [&](int j, UT v) {
+      store<T, NT>(out + j * lanes<UT>::N, alp_value<T>(v, base, fact, frac));
+    });
+  }
+};
+
+// Bit (j - J0) set when slot j decodes (before patching) to `target`.
+template <typename T>
+struct AlpMatchPolicy {
+  using UT = typename uint_of<T>::type;
+  const UT *in;
+  UT base;
+  UT fact;
+  T frac;
+  T target;
+  uint64_t mask;
+  template <int BW, int J0, int NJ>
+  __device__ __forceinline__ void run() {
+    unpack_slots<UT, BW, J0, NJ>(in, [&](int j, UT v) {
+      mask |= uint64_t(alp_value<T>(v, base, fact, frac) == target) << (j - J0);
+    });
+  }
+};
+
+// Calls f(position, value) for every exception of vector `vec` that lies in
+// lane `lane` and in slots [j0, j0 + nj).
+template <typename T, int LAYOUT, class F>
+__device__ __forceinline__ void for_own_exceptions(const AlpArgs<T> &a, uint64_t vec, int lane,
+                                                   int j0, int nj, F &&f) {
+  constexpr int NL = lanes<typename uint_of<T>::type>::N;
+  const uint64_t eo = a.exceptions_offsets[vec];
+  if constexpr (LAYOUT == ALP_GALP) {
+    const unsigned oc = a.offsets_counts[vec * NL + lane];
+    const unsigned count = oc >> 10;
+    const uint64_t first = eo + (oc & 0x3FFu);
+    for (unsigned k = 0; k < count; ++k) {
+      const unsigned pos = a.positions[first + k];
+      const int slot = int(pos) / NL;
+      if (slot >= j0 && slot < j0 + nj) f(pos, a.exceptions[first + k]);
+    }
+  } else {
+    const unsigned count = a.counts[vec];
+    for (unsigned k = 0; k < count; ++k) {
+      const unsigned pos = a.positions[eo + k];
+      const int slot = int(pos) / NL;
+      if (int(pos % NL) == lane && slot >= j0 && slot < j0 + nj) f(pos, a.exceptions[eo + k]);
+    }
+  }
+}
+
+template <typename T, int S, bool NT, int LAYOUT, bool PATCH>
+__global__ __launch_bounds__(BLOCK) void k_alp_decompress(AlpArgs<T> a, T *__restrict out) {
+  using UT = typename uint_of<T>::type;
+  using L = lanes<UT>;
+  const uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE;
+  const UnitMap<UT, S> u = map_unit<UT, S>(wave);
+  if (u.vec >= a.ffor.n_vecs) return;
+  T *vout = out + u.vec * VALUES_PER_VECTOR;
+  AlpDecompressPolicy<T, NT> pol{a.ffor.packed + a.ffor.vector_offsets[u.vec] + u.lane,
+                                 vout + u.lane, a.ffor.bases[u.vec],
+                                 alp_factor<T>(a.factor_indices[u.vec]),
+                                 alp_fraction<T>(a.fraction_indices[u.vec])};
+  const int j0 = u.part * (L::W / S);
+  if (!dispatch_bw<UT, S>(pol, a.ffor.bit_widths[u.vec], u.part)) {
+    for (int j = j0; j < j0 + L::W / S; ++j) vout[u.lane + j * L::N] = T(0);
+  }
+  if constexpr (PATCH) {
+    for_own_exceptions<T, LAYOUT>(a, u.vec, u.lane, j0, L::W / S,
+                                  [&](unsigned pos, T e) { store<T, NT>(vout + pos, e); });
+  }
+}
+
+// Slot-match mask of one (vector, lane) after patching; exceptions replace
+// the decoded value at their slot.
+template <typename T, int LAYOUT, bool PATCH>
+__device__ __forceinline__ uint64_t alp_lane_match(const AlpArgs<T> &a, uint64_t vec, int lane,
+                                                   T target) {
+  using UT = typename uint_of<T>::type;
+  using L = lanes<UT>;
+  AlpMatchPolicy<T> pol{a.ffor.packed + a.ffor.vector_offsets[vec] + lane, a.ffor.bases[vec],
+                        alp_factor<T>(a.factor_indices[vec]),
+                        alp_fraction<T>(a.fraction_indices[vec]), target, 0};
+  if (!dispatch_bw<UT, 1>(pol, a.ffor.bit_widths[vec], 0)) {
+    constexpr uint64_t FULL = L::W == 64 ? ~uint64_t(0) : ((uint64_t(1) << L::W) - 1);
+    pol.mask = T(0) == target ? FULL : 0;
+  }
+  uint64_t m = pol.mask;
+  if constexpr (PATCH) {
+    for_own_exceptions<T, LAYOUT>(a, vec, lane, 0, L::W, [&](unsigned pos, T e) {
+      const uint64_t bit = uint64_t(1) << (pos / L::N);
+      m = (e == target) ? (m | bit) : (m & ~bit);
+    });
+  }
+  return m;
+}
+
+template <typename T, int LAYOUT, bool PATCH>
+__global__ __launch_bounds__(BLOCK) void k_alp_any(AlpArgs<T> a, T magic, uint8_t *d_found,
+                                                  uint64_t n_waves, Scratch s) {
+  using UT = typename uint_of<T>::type;
+  bool found = false;
+  const uint64_t stride = uint64_t(gridDim.x) * (BLOCK / WAVE);
+  for (uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE; wave < n_waves;
+       wave += stride) {
+    const UnitMap<UT, 1> u = map_unit<UT, 1>(wave);
+    if (u.vec < a.ffor.n_vecs) found |= alp_lane_match<T, LAYOUT, PATCH>(a, u.vec, u.lane, magic) != 0;
+  }
+  grid_reduce2<RedOp::Or>(found, 0, s, [&](uint64_t f, uint64_t) { *d_found = f ? 1 : 0; });
+}
+
+template <typename T, int LAYOUT, bool PATCH>
+__global__ __launch_bounds__(BLOCK) void k_alp_multi(MultiAlpArgs<T> a, T value, uint8_t *d_found,
+                                                    uint64_t n_waves, Scratch s) {
+  using UT = typename uint_of<T>::type;
+  using L = lanes<UT>;
+  constexpr uint64_t FULL = L::W == 64 ? ~uint64_t(0) : ((uint64_t(1) << L::W) - 1);
+  bool found = false;
+  const uint64_t stride = uint64_t(gridDim.x) * (BLOCK / WAVE);
+  for (uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE; wave < n_waves;
+       wave += stride) {
+    const UnitMap<UT, 1> u = map_unit<UT, 1>(wave);
+    if (u.vec < a.col[0].ffor.n_vecs) {
+      uint64_t all = FULL;
+      for (uint32_t c = 0; c < a.n_cols; ++c)
+        all &= alp_lane_match<T, LAYOUT, PATCH>(a.col[c], u.vec, u.lane, value);
+      found |= all == FULL;
+    }
+  }
+  grid_reduce2<RedOp::Or>(found, 0, s, [&](uint64_t f, uint64_t) { *d_found = f ? 1 : 0; });
+}
+
+template <typename T, int S, bool NT, int LAYOUT, bool PATCH>
+int AlpDecompress<T, S, NT, LAYOUT, PATCH>::launch(const AlpArgs<T> &a, T *out,
+                                                   hipStream_t stream) {
+  using UT = typename uint_of<T>::type;
+  if (a.ffor.n_vecs == 0) return FLSGPU_OK;
+  const uint64_t blocks = (n_waves_for<UT, S>(a.ffor.n_vecs) + (BLOCK / WAVE) - 1) / (BLOCK / WAVE);
+  if (blocks > 0x7FFFFFFFull) return set_error(FLSGPU_EINVAL, "column too large");
+  hipLaunchKernelGGL((k_alp_decompress<T, S, NT, LAYOUT, PATCH>), dim3(unsigned(blocks)),
+                     dim3(BLOCK), 0, stream, a, out);
+  return check_hip(hipGetLastError(), "k_alp_decompress launch");
+}
+
+template <typename T, int LAYOUT, bool PATCH>
+int AlpAny<T, LAYOUT, PATCH>::launch(const AlpArgs<T> &a, T magic, uint8_t *d_found,
+                                     hipStream_t stream) {
+  using UT = typename uint_of<T>::type;
+  Scratch s;
+  int rc = get_scratch(&s);
+  if (rc) return rc;
+  const uint64_t n_waves = n_waves_for<UT, 1>(a.ffor.n_vecs);
+  hipLaunchKernelGGL((k_alp_any<T, LAYOUT, PATCH>), dim3(reduce_grid(n_waves)), dim3(BLOCK), 0,
+                     stream, a, magic, d_found, n_waves, s);
+  return check_hip(hipGetLastError(), "k_alp_any launch");
+}
+
+template <typename T, int LAYOUT, bool PATCH>
+int AlpMulti<T, LAYOUT, PATCH>::launch(const MultiAlpArgs<T> &a, T value, uint8_t *d_found,
+                                       hipStream_t stream) {
+  using UT = typename uint_of<T>::type;
+  Scratch s;
+  int rc = get_scratch(&s);
+  if (rc) return rc;
+  const uint64_t n_waves = n_waves_for<UT, 1>(a.col[0].ffor.n_vecs);
+  hipLaunchKernelGGL((k_alp_multi<T, LAYOUT, PATCH>), dim3(reduce_grid(n_waves)), dim3(BLOCK), 0,
+                     stream, a, value, d_found, n_waves, s);
+  return check_hip(hipGetLastError(), "k_alp_multi launch");
+}
+
+}  // namespace flsgpu_hip

@@ -1,0 +1,112 @@
+// ALP-RD decode kernels for gfx950 (the reference has only a CPU decoder of
+// unpacked parts, src/alp/rd.hpp:146-172, with no caller).
+//
+//   out[i] = ((UT)dictionary[left_index[i]] << right_bw) | right[i]
+//   exceptions: out[pos] = ((UT)exception_left << right_bw) | right[pos]
+//
+// Right parts are FastLanes words of the float's width packed at right_bw
+// (16 compile-time cases, W-16 .. W-1); left dictionary indices are uint16
+// FastLanes words packed at left_bw (1..3 bits). The thread that owns lane l
+// of the W-bit layout needs, for slot j, the uint16 lane l + N_LANES*(j % Q)
+// at slot j / Q (Q = 64 / N_LANES); its Q uint16 lanes hold at most
+// 16 * 3 = 48 bits each, so each is loaded once into one 64-bit register. The
+// dictionary, pre-shifted by right_bw, sits in LDS (8 entries per block).
+#pragma once
+
+#include "alprd_impl.hpp"
+#include "kernels.hpp"
+
+namespace flsgpu_hip {
+
+template <typename T, bool NT>
+struct RdPolicy {
+  using UT = typename uint_of<T>::type;
+  static constexpr int NL = lanes<UT>::N;
+  static constexpr int Q = 64 / NL;
+  const UT *rin;
+  T *out;
+  const UT *dshift;  // LDS, 8 entries
+  uint64_t lstream[Q];
+  unsigned lbw;
+  template <int BW, int J0, int NJ>
+  __device__ __forceinline__ void run() {
+    const uint64_t lmask = (uint64_t(1) << lbw) - 1;
+    unpack_slots<UT, BW, J0, NJ>(rin, [&](int j, UT r) {
+      const unsigned idx = unsigned(lstream[j % Q] >> ((j / Q) * lbw)) & unsigned(lmask);
+      const UT v = dshift[idx] | r;
+      store<T, NT>(out + j * NL, __builtin_bit_cast(T, v));
+    });
+  }
+};
+
+template <typename UT, class Policy>
+__device__ __forceinline__ bool dispatch_right_bw(Policy &pol, int bw) {
+  constexpr int W = lanes<UT>::W;
+  for (;;) {
+    const int b = __builtin_amdgcn_readfirstlane(bw);
+    if (b == bw) {
+      switch (b - (W - 16)) {
+#define FLSGPU_RD_CASE(K) \
+  case K: pol.template run<W - 16 + K, 0, W>(); return true;
+        FLSGPU_RD_CASE(0) FLSGPU_RD_CASE(1) FLSGPU_RD_CASE(2) FLSGPU_RD_CASE(3)
+        FLSGPU_RD_CASE(4) FLSGPU_RD_CASE(5) FLSGPU_RD_CASE(6) FLSGPU_RD_CASE(7)
+        FLSGPU_RD_CASE(8) FLSGPU_RD_CASE(9) FLSGPU_RD_CASE(10) FLSGPU_RD_CASE(11)
+        FLSGPU_RD_CASE(12) FLSGPU_RD_CASE(13) FLSGPU_RD_CASE(14) FLSGPU_RD_CASE(15)
+#undef FLSGPU_RD_CASE
+        default: return false;
+      }
+    }
+  }
+}
+
+template <typename T, bool NT>
+__global__ __launch_bounds__(BLOCK) void k_alprd_decompress(AlprdArgs<T> a, T *__restrict out) {
+  using UT = typename uint_of<T>::type;
+  using L = lanes<UT>;
+  constexpr int Q = 64 / L::N;
+  __shared__ UT dshift[8];
+  if (threadIdx.x < 8)
+    dshift[threadIdx.x] = threadIdx.x < a.dict_size ? UT(UT(a.dict[threadIdx.x]) << a.right_bw) : UT(0);
+  __syncthreads();
+  const uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE;
+  const UnitMap<UT, 1> u = map_unit<UT, 1>(wave);
+  if (u.vec >= a.n_vecs) return;
+  T *vout = out + u.vec * VALUES_PER_VECTOR;
+  const UT *rin = a.right + u.vec * uint64_t(a.right_bw) * L::N + u.lane;
+  RdPolicy<T, NT> pol;
+  pol.rin = rin;
+  pol.out = vout + u.lane;
+  pol.dshift = dshift;
+  pol.lbw = a.left_bw;
+  const uint16_t *lin = a.left + u.vec * uint64_t(a.left_bw) * 64;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    uint64_t s = 0;
+    for (unsigned r = 0; r < a.left_bw; ++r)
+      s |= uint64_t(lin[u.lane + L::N * q + 64 * r]) << (16 * r);
+    pol.lstream[q] = s;
+  }
+  if (!dispatch_right_bw<UT>(pol, int(a.right_bw))) return;  // rejected on the host
+  const uint64_t eo = a.exceptions_offsets[u.vec];
+  const unsigned count = a.counts[u.vec];
+  for (unsigned k = 0; k < count; ++k) {
+    const unsigned pos = a.positions[eo + k];
+    if (int(pos % L::N) != u.lane) continue;
+    const UT right = extract_runtime<UT>(rin, int(a.right_bw), int(pos / L::N));
+    const UT v = UT(UT(a.exceptions[eo + k]) << a.right_bw) | right;
+    store<T, NT>(vout + pos, __builtin_bit_cast(T, v));
+  }
+}
+
+template <typename T, bool NT>
+int AlprdDecompress<T, NT>::launch(const AlprdArgs<T> &a, T *out, hipStream_t stream) {
+  using UT = typename uint_of<T>::type;
+  if (a.n_vecs == 0) return FLSGPU_OK;
+  const uint64_t blocks = (n_waves_for<UT, 1>(a.n_vecs) + (BLOCK / WAVE) - 1) / (BLOCK / WAVE);
+  if (blocks > 0x7FFFFFFFull) return set_error(FLSGPU_EINVAL, "column too large");
+  hipLaunchKernelGGL((k_alprd_decompress<T, NT>), dim3(unsigned(blocks)), dim3(BLOCK), 0, stream,
+                     a, out);
+  return check_hip(hipGetLastError(), "k_alprd_decompress launch");
+}
+
+}  // namespace flsgpu_hip

@@ -1,0 +1,174 @@
+// Library state behind the C ABI: error reporting, per-device scratch for
+// grid reductions, device memory helpers and the autotuner table.
+//
+// Replaces the reference's CUDA_SAFE_CALL -> exit(EXIT_FAILURE)
+// (src/flsgpu/host-utils.cuh:12-32) with status codes, and GPUArray /
+// copy_to_device (host-utils.cuh:42-111, src/flsgpu/structs.cuh:103-229) with
+// flsgpu_upload / flsgpu_free.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+
+#include "common.hpp"
+
+namespace flsgpu_hip {
+
+static thread_local char g_err[512] = "";
+
+int set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int check_hip(hipError_t e, const char *what) {
+  if (e == hipSuccess) return FLSGPU_OK;
+  return set_error(FLSGPU_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_mu;
+static Scratch g_scratch[MAX_DEVICES];
+static int g_cus[MAX_DEVICES];
+
+static int current_device(int *dev) {
+  int rc = check_hip(hipGetDevice(dev), "hipGetDevice");
+  if (rc) return rc;
+  if (*dev < 0 || *dev >= MAX_DEVICES) return set_error(FLSGPU_EINVAL, "device %d out of range", *dev);
+  return FLSGPU_OK;
+}
+
+static int init_device_locked(int dev) {
+  if (g_scratch[dev].partials) return FLSGPU_OK;
+  hipDeviceProp_t prop;
+  int rc = check_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+  if (rc) return rc;
+  g_cus[dev] = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  void *p = nullptr;
+  const size_t bytes = sizeof(uint64_t) * 2 * MAX_REDUCE_BLOCKS + 256;
+  if ((rc = check_hip(hipMalloc(&p, bytes), "hipMalloc(scratch)"))) return rc;
+  if ((rc = check_hip(hipMemset(p, 0, bytes), "hipMemset(scratch)"))) {
+    (void)hipFree(p);
+    return rc;
+  }
+  g_scratch[dev].partials = static_cast<uint64_t *>(p);
+  g_scratch[dev].counter =
+      reinterpret_cast<unsigned *>(static_cast<char *>(p) + sizeof(uint64_t) * 2 * MAX_REDUCE_BLOCKS);
+  return FLSGPU_OK;
+}
+
+int get_scratch(Scratch *s) {
+  int dev;
+  int rc = current_device(&dev);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if ((rc = init_device_locked(dev))) return rc;
+  *s = g_scratch[dev];
+  return FLSGPU_OK;
+}
+
+int num_cus() {
+  int dev;
+  if (current_device(&dev)) return 256;
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (init_device_locked(dev)) return 256;
+  return g_cus[dev];
+}
+
+unsigned reduce_grid(uint64_t n_waves) {
+  const uint64_t blocks = (n_waves + (BLOCK / WAVE) - 1) / (BLOCK / WAVE);
+  const uint64_t want = uint64_t(num_cus()) * 8;
+  const uint64_t cap = want < MAX_REDUCE_BLOCKS ? want : MAX_REDUCE_BLOCKS;
+  return unsigned(blocks < 1 ? 1 : (blocks < cap ? blocks : cap));
+}
+
+// Autotuner table: [dtype 32/64][bit width 0..64, 255 = column default].
+static int g_tuning[2][66];
+static std::once_flag g_tuning_once;
+static void tuning_defaults() {
+  for (int d = 0; d < 2; ++d)
+    for (int b = 0; b < 66; ++b) g_tuning[d][b] = FLSGPU_VARIANT_LANE;
+}
+
+}  // namespace flsgpu_hip
+
+using namespace flsgpu_hip;
+
+extern "C" {
+
+const char *flsgpu_last_error(void) { return g_err; }
+
+int flsgpu_init(int device) {
+  int rc = check_hip(hipSetDevice(device), "hipSetDevice");
+  if (rc) return rc;
+  if (device < 0 || device >= MAX_DEVICES) return set_error(FLSGPU_EINVAL, "bad device");
+  std::lock_guard<std::mutex> lk(g_mu);
+  return init_device_locked(device);
+}
+
+int flsgpu_set_tuning(uint32_t dtype_bits, uint32_t bit_width, int variant) {
+  std::call_once(g_tuning_once, tuning_defaults);
+  if (dtype_bits != 32 && dtype_bits != 64) return set_error(FLSGPU_EINVAL, "dtype_bits must be 32 or 64");
+  if (variant <= FLSGPU_VARIANT_AUTO || variant >= FLSGPU_VARIANT_COUNT)
+    return set_error(FLSGPU_EINVAL, "variant %d out of range", variant);
+  const int d = dtype_bits == 64;
+  if (bit_width == 255) {
+    g_tuning[d][65] = variant;
+  } else {
+    if (bit_width > dtype_bits) return set_error(FLSGPU_EINVAL, "bit width %u > %u", bit_width, dtype_bits);
+    g_tuning[d][bit_width] = variant;
+  }
+  return FLSGPU_OK;
+}
+
+int flsgpu_get_tuning(uint32_t dtype_bits, uint32_t bit_width) {
+  std::call_once(g_tuning_once, tuning_defaults);
+  const int d = dtype_bits == 64;
+  if (bit_width == 255) return g_tuning[d][65];
+  if (bit_width > 64) return FLSGPU_VARIANT_LANE;
+  return g_tuning[d][bit_width];
+}
+
+int flsgpu_upload(void **d_ptr, const void *h_ptr, size_t bytes, size_t pad_bytes,
+                  flsgpu_stream_t stream) {
+  if (!d_ptr || (!h_ptr && bytes)) return set_error(FLSGPU_EINVAL, "null pointer");
+  void *p = nullptr;
+  const size_t total = bytes + pad_bytes;
+  int rc = check_hip(hipMalloc(&p, total ? total : 1), "hipMalloc");
+  if (rc) return rc;
+  if (bytes && (rc = check_hip(hipMemcpyAsync(p, h_ptr, bytes, hipMemcpyHostToDevice, stream),
+                               "hipMemcpyAsync H2D"))) {
+    (void)hipFree(p);
+    return rc;
+  }
+  if (pad_bytes && (rc = check_hip(hipMemsetAsync(static_cast<char *>(p) + bytes, 0, pad_bytes, stream),
+                                   "hipMemsetAsync"))) {
+    (void)hipFree(p);
+    return rc;
+  }
+  if ((rc = check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize"))) {
+    (void)hipFree(p);
+    return rc;
+  }
+  *d_ptr = p;
+  return FLSGPU_OK;
+}
+
+int flsgpu_download(void *h_ptr, const void *d_ptr, size_t bytes, flsgpu_stream_t stream) {
+  if (!bytes) return FLSGPU_OK;
+  if (!h_ptr || !d_ptr) return set_error(FLSGPU_EINVAL, "null pointer");
+  int rc = check_hip(hipMemcpyAsync(h_ptr, d_ptr, bytes, hipMemcpyDeviceToHost, stream),
+                     "hipMemcpyAsync D2H");
+  if (rc) return rc;
+  return check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+int flsgpu_free(void *d_ptr) {
+  if (!d_ptr) return FLSGPU_OK;
+  return check_hip(hipFree(d_ptr), "hipFree");
+}
+
+}  // extern "C"

@@ -1,0 +1,280 @@
+// FastLanes bit-unpacking / FFOR kernels for gfx950: decompress, "any ==
+// magic" query, fused filter + sum, the synthetic compute kernel and the
+// multi-column query. Included only by fls_inst.hip.
+//
+// Reference behaviour: src/engine/kernels.cuh:13-153 (decompress/query/compute
+// kernels and host launchers), src/flsgpu/fls.cuh:17-41 (BP/FFOR functors:
+// value + base, wrapping mod 2^W), src/engine/device-utils.cuh:109-143
+// (output index lane + N_LANES*j + 1024*v, MagicChecker),
+// src/generated-bindings/multi-column-device-kernels.cuh:10-443.
+#pragma once
+
+#include "fls_impl.hpp"
+#include "kernels.hpp"
+
+namespace flsgpu_hip {
+
+// ---------------------------------------------------------------------------
+// Policies: what happens to each unpacked value.
+// ---------------------------------------------------------------------------
+template <typename UT, bool NT>
+struct DecompressPolicy {
+  const UT *in;
+  UT *out;
+  UT base;
+  template <int BW, int J0, int NJ>
+  __device__ __forceinline__ void run() {
+    unpack_slots<UT, BW, J0, NJ>(in, [&](int j, UT v) {
+      store<UT, NT>(out + j * lanes<UT>::N, UT(v + base));
+    });
+  }
+};
+
+template <typename UT>
+struct AnyPolicy {
+  const UT *in;
+  UT base;
+  UT magic;
+  bool found;
+  template <int BW, int J0, int NJ>
+  __device__ __forceinline__ void run() {
+    unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT v) { found |= UT(v + base) == magic; });
+  }
+};
+
+template <typename UT, int PRED>
+struct FilterSumPolicy {
+  const UT *in;
+  UT base;
+  UT lo, hi;
+  uint64_t count, sum;
+  template <int BW, int J0, int NJ>
+  __device__ __forceinline__ void run() {
+    if constexpr (PRED == FLSGPU_PRED_OFFSET_LOWER_HALF) {
+      if constexpr (BW > 0) {  // no row passes at bit width 0
+        constexpr UT HALF = UT(UT(1) << (BW - 1));
+        unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT off) {
+          const bool pass = off < HALF;
+          count += pass;
+          sum += pass ? uint64_t(UT(off + base)) : 0;
+        });
+      }
+    } else {
+      unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT off) {
+        const UT v = UT(off + base);
+        const bool pass = (v >= lo) & (v < hi);
+        count += pass;
+        sum += pass ? uint64_t(v) : 0;
+      });
+    }
+  }
+};
+
+template <typename UT>
+struct ComputePolicy {
+  const UT *in;
+  UT base;
+  uint32_t reps;
+  UT runtime_zero;
+  bool found;
+  template <int BW, int J0, int NJ>
+  __device__ __forceinline__ void run() {
+    unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT v) {
+      UT x = UT(v + base);
+      for (uint32_t k = 0; k < reps; ++k) {
+        x *= UT(3);
+        x <<= 3;
+        x += UT(3);
+        x >>= 3;
+        x &= runtime_zero;
+      }
+      found |= x == UT(1);
+    });
+  }
+};
+
+// Bit j - J0 of `mask` is set when slot j of this lane equals `value`.
+template <typename UT>
+struct MatchMaskPolicy {
+  const UT *in;
+  UT base;
+  UT value;
+  uint64_t mask;
+  template <int BW, int J0, int NJ>
+  __device__ __forceinline__ void run() {
+    unpack_slots<UT, BW, J0, NJ>(in, [&](int j, UT v) {
+      mask |= uint64_t(UT(v + base) == value) << (j - J0);
+    });
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+template <typename UT, int S, bool NT>
+__global__ __launch_bounds__(BLOCK) void k_ffor_decompress(FforArgs<UT> a, UT *__restrict out) {
+  using L = lanes<UT>;
+  const uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE;
+  const UnitMap<UT, S> u = map_unit<UT, S>(wave);
+  if (u.vec >= a.n_vecs) return;
+  DecompressPolicy<UT, NT> pol{a.packed + a.vector_offsets[u.vec] + u.lane,
+                               out + u.vec * VALUES_PER_VECTOR + u.lane,
+                               a.bases ? a.bases[u.vec] : UT(0)};
+  if (!dispatch_bw<UT, S>(pol, a.bit_widths[u.vec], u.part)) {
+    // Bit width above W: the vector decodes as zeros.
+    for (int j = u.part * (L::W / S); j < (u.part + 1) * (L::W / S); ++j)
+      out[u.vec * VALUES_PER_VECTOR + u.lane + j * L::N] = UT(0);
+  }
+}
+
+template <typename UT, int S>
+__global__ __launch_bounds__(BLOCK) void k_ffor_any(FforArgs<UT> a, UT magic, uint8_t *d_found,
+                                                   uint64_t n_waves, Scratch s) {
+  bool found = false;
+  const uint64_t stride = uint64_t(gridDim.x) * (BLOCK / WAVE);
+  for (uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE; wave < n_waves;
+       wave += stride) {
+    const UnitMap<UT, S> u = map_unit<UT, S>(wave);
+    if (u.vec < a.n_vecs) {
+      AnyPolicy<UT> pol{a.packed + a.vector_offsets[u.vec] + u.lane,
+                        a.bases ? a.bases[u.vec] : UT(0), magic, false};
+      if (dispatch_bw<UT, S>(pol, a.bit_widths[u.vec], u.part)) found |= pol.found;
+      else found |= magic == UT(0);
+    }
+  }
+  grid_reduce2<RedOp::Or>(found, 0, s, [&](uint64_t f, uint64_t) { *d_found = f ? 1 : 0; });
+}
+
+template <typename UT, int S, int PRED>
+__global__ __launch_bounds__(BLOCK) void k_ffor_filter_sum(FforArgs<UT> a, UT lo, UT hi,
+                                                          uint64_t *d_result, uint64_t n_waves,
+                                                          Scratch s) {
+  uint64_t count = 0, sum = 0;
+  const uint64_t stride = uint64_t(gridDim.x) * (BLOCK / WAVE);
+  for (uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE; wave < n_waves;
+       wave += stride) {
+    const UnitMap<UT, S> u = map_unit<UT, S>(wave);
+    if (u.vec < a.n_vecs) {
+      FilterSumPolicy<UT, PRED> pol{a.packed + a.vector_offsets[u.vec] + u.lane,
+                                    a.bases ? a.bases[u.vec] : UT(0), lo, hi, 0, 0};
+      dispatch_bw<UT, S>(pol, a.bit_widths[u.vec], u.part);
+      count += pol.count;
+      sum += pol.sum;
+    }
+  }
+  grid_reduce2<RedOp::Add>(count, sum, s, [&](uint64_t c, uint64_t v) {
+    d_result[0] = c;
+    d_result[1] = v;
+  });
+}
+
+template <typename UT>
+__global__ __launch_bounds__(BLOCK) void k_ffor_compute(FforArgs<UT> a, uint32_t reps,
+                                                       UT runtime_zero, uint8_t *d_found,
+                                                       uint64_t n_waves, Scratch s) {
+  bool found = false;
+  const uint64_t stride = uint64_t(gridDim.x) * (BLOCK / WAVE);
+  for (uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE; wave < n_waves;
+       wave += stride) {
+    const UnitMap<UT, 1> u = map_unit<UT, 1>(wave);
+    if (u.vec < a.n_vecs) {
+      ComputePolicy<UT> pol{a.packed + a.vector_offsets[u.vec] + u.lane,
+                            a.bases ? a.bases[u.vec] : UT(0), reps, runtime_zero, false};
+      dispatch_bw<UT, 1>(pol, a.bit_widths[u.vec], 0);
+      found |= pol.found;
+    }
+  }
+  grid_reduce2<RedOp::Or>(found, 0, s, [&](uint64_t f, uint64_t) { *d_found = f ? 1 : 0; });
+}
+
+// One thread per (vector, lane): the answer is 1 iff some lane has every slot
+// equal to `value` in every column.
+template <typename UT>
+__global__ __launch_bounds__(BLOCK) void k_ffor_multi(MultiFforArgs<UT> a, UT value,
+                                                     uint8_t *d_found, uint64_t n_waves,
+                                                     Scratch s) {
+  using L = lanes<UT>;
+  constexpr uint64_t FULL = L::W == 64 ? ~uint64_t(0) : ((uint64_t(1) << L::W) - 1);
+  bool found = false;
+  const uint64_t stride = uint64_t(gridDim.x) * (BLOCK / WAVE);
+  const uint64_t n_vecs = a.col[0].n_vecs;
+  for (uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE; wave < n_waves;
+       wave += stride) {
+    const UnitMap<UT, 1> u = map_unit<UT, 1>(wave);
+    if (u.vec < n_vecs) {
+      uint64_t all = FULL;
+      for (uint32_t c = 0; c < a.n_cols; ++c) {
+        const FforArgs<UT> &k = a.col[c];
+        MatchMaskPolicy<UT> pol{k.packed + k.vector_offsets[u.vec] + u.lane,
+                                k.bases ? k.bases[u.vec] : UT(0), value, 0};
+        if (!dispatch_bw<UT, 1>(pol, k.bit_widths[u.vec], 0))
+          pol.mask = value == UT(0) ? FULL : 0;
+        all &= pol.mask;
+      }
+      found |= all == FULL;
+    }
+  }
+  grid_reduce2<RedOp::Or>(found, 0, s, [&](uint64_t f, uint64_t) { *d_found = f ? 1 : 0; });
+}
+
+// ---------------------------------------------------------------------------
+// Launchers (explicitly instantiated in fls_inst.hip)
+// ---------------------------------------------------------------------------
+template <typename UT, int S, bool NT>
+int FlsDecompress<UT, S, NT>::launch(const FforArgs<UT> &a, UT *out, hipStream_t stream) {
+  if (a.n_vecs == 0) return FLSGPU_OK;
+  const uint64_t blocks = (n_waves_for<UT, S>(a.n_vecs) + (BLOCK / WAVE) - 1) / (BLOCK / WAVE);
+  if (blocks > 0x7FFFFFFFull) return set_error(FLSGPU_EINVAL, "column too large");
+  hipLaunchKernelGGL((k_ffor_decompress<UT, S, NT>), dim3(unsigned(blocks)), dim3(BLOCK), 0,
+                     stream, a, out);
+  return check_hip(hipGetLastError(), "k_ffor_decompress launch");
+}
+
+template <typename UT, int S>
+int FlsAny<UT, S>::launch(const FforArgs<UT> &a, UT magic, uint8_t *d_found, hipStream_t stream) {
+  Scratch s;
+  int rc = get_scratch(&s);
+  if (rc) return rc;
+  const uint64_t n_waves = n_waves_for<UT, S>(a.n_vecs);
+  hipLaunchKernelGGL((k_ffor_any<UT, S>), dim3(reduce_grid(n_waves)), dim3(BLOCK), 0, stream, a,
+                     magic, d_found, n_waves, s);
+  return check_hip(hipGetLastError(), "k_ffor_any launch");
+}
+
+template <typename UT, int S, int PRED>
+int FlsFilterSum<UT, S, PRED>::launch(const FforArgs<UT> &a, UT lo, UT hi, uint64_t *d_result,
+                                      hipStream_t stream) {
+  Scratch s;
+  int rc = get_scratch(&s);
+  if (rc) return rc;
+  const uint64_t n_waves = n_waves_for<UT, S>(a.n_vecs);
+  hipLaunchKernelGGL((k_ffor_filter_sum<UT, S, PRED>), dim3(reduce_grid(n_waves)), dim3(BLOCK), 0,
+                     stream, a, lo, hi, d_result, n_waves, s);
+  return check_hip(hipGetLastError(), "k_ffor_filter_sum launch");
+}
+
+template <typename UT>
+int FlsCompute<UT>::launch(const FforArgs<UT> &a, uint32_t reps, uint8_t *d_found,
+                           hipStream_t stream) {
+  Scratch s;
+  int rc = get_scratch(&s);
+  if (rc) return rc;
+  const uint64_t n_waves = n_waves_for<UT, 1>(a.n_vecs);
+  hipLaunchKernelGGL((k_ffor_compute<UT>), dim3(reduce_grid(n_waves)), dim3(BLOCK), 0, stream, a,
+                     reps, UT(0), d_found, n_waves, s);
+  return check_hip(hipGetLastError(), "k_ffor_compute launch");
+}
+
+template <typename UT>
+int FlsMulti<UT>::launch(const MultiFforArgs<UT> &a, UT value, uint8_t *d_found,
+                         hipStream_t stream) {
+  Scratch s;
+  int rc = get_scratch(&s);
+  if (rc) return rc;
+  const uint64_t n_waves = n_waves_for<UT, 1>(a.col[0].n_vecs);
+  hipLaunchKernelGGL((k_ffor_multi<UT>), dim3(reduce_grid(n_waves)), dim3(BLOCK), 0, stream, a,
+                     value, d_found, n_waves, s);
+  return check_hip(hipGetLastError(), "k_ffor_multi launch");
+}
+
+}  // namespace flsgpu_hip

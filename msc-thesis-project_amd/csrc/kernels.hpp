@@ -1,0 +1,134 @@
+// Work mapping and cross-block reduction shared by all decoder kernels.
+//
+// Mapping (replaces FillWarpMapping / FillWarpThreadblockMapping of
+// src/engine/device-utils.cuh:24-99): one FastLanes lane per thread, so a
+// 64-wide wavefront covers VPW = 64 / N_LANES vectors (2 for 32-bit, 4 for
+// 64-bit types). A lane may be split over S waves ("parts"); the part index
+// is wave-uniform, so every branch on it is scalar. Vectors are never skipped:
+// the reference's integer-division grid drops trailing vectors
+// (device-utils.cuh:35); here the grid is rounded up and the tail masked.
+#pragma once
+
+#include "unpack.hpp"
+
+namespace flsgpu_hip {
+
+template <typename UT, int S>
+struct UnitMap {
+  uint64_t vec;
+  int lane;
+  int part;
+};
+
+template <typename UT, int S>
+__device__ __forceinline__ UnitMap<UT, S> map_unit(uint64_t wave_index) {
+  using L = lanes<UT>;
+  UnitMap<UT, S> u;
+  u.part = __builtin_amdgcn_readfirstlane(int(wave_index % S));
+  const uint64_t group = wave_index / S;
+  const int t = int(threadIdx.x % WAVE);
+  u.lane = t % L::N;
+  u.vec = group * L::VPW + uint64_t(t / L::N);
+  return u;
+}
+
+template <typename UT, int S>
+__host__ __device__ constexpr uint64_t n_waves_for(uint64_t n_vecs) {
+  return ((n_vecs + lanes<UT>::VPW - 1) / lanes<UT>::VPW) * S;
+}
+
+enum class RedOp { Add, Or, And };
+
+template <RedOp OP>
+__device__ __forceinline__ uint64_t red_combine(uint64_t a, uint64_t b) {
+  if constexpr (OP == RedOp::Add) return a + b;
+  else if constexpr (OP == RedOp::Or) return a | b;
+  else return a & b;
+}
+
+template <RedOp OP>
+__device__ __forceinline__ uint64_t red_identity() {
+  return OP == RedOp::And ? ~uint64_t(0) : uint64_t(0);
+}
+
+template <RedOp OP>
+__device__ __forceinline__ uint64_t wave_reduce(uint64_t v) {
+#pragma unroll
+  for (int off = WAVE / 2; off > 0; off >>= 1) v = red_combine<OP>(v, __shfl_xor(v, off, WAVE));
+  return v;
+}
+
+// Grid-wide reduction of two values: every block reduces in LDS and writes a
+// partial; the last block to finish (atomic ticket) reduces the partials in a
+// fixed order and calls done(total0, total1) from thread 0, then re-arms the
+// ticket for the next launch. Deterministic for any grid size.
+template <RedOp OP, class Done>
+__device__ __forceinline__ void grid_reduce2(uint64_t v0, uint64_t v1, Scratch s, Done &&done) {
+  __shared__ uint64_t sh[BLOCK / WAVE][2];
+  __shared__ int am_last;
+  v0 = wave_reduce<OP>(v0);
+  v1 = wave_reduce<OP>(v1);
+  const int w = threadIdx.x / WAVE;
+  if ((threadIdx.x % WAVE) == 0) {
+    sh[w][0] = v0;
+    sh[w][1] = v1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t a = red_identity<OP>(), b = red_identity<OP>();
+    for (int i = 0; i < BLOCK / WAVE; ++i) {
+      a = red_combine<OP>(a, sh[i][0]);
+      b = red_combine<OP>(b, sh[i][1]);
+    }
+    __hip_atomic_store(&s.partials[2 * blockIdx.x], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&s.partials[2 * blockIdx.x + 1], b, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    const unsigned prev = atomicAdd(s.counter, 1u);
+    am_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!am_last) return;
+  __threadfence();
+  uint64_t a = red_identity<OP>(), b = red_identity<OP>();
+  for (unsigned i = threadIdx.x; i < gridDim.x; i += BLOCK) {
+    a = red_combine<OP>(
+        a, __hip_atomic_load(&s.partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    b = red_combine<OP>(
+        b, __hip_atomic_load(&s.partials[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  }
+  a = wave_reduce<OP>(a);
+  b = wave_reduce<OP>(b);
+  __syncthreads();
+  if ((threadIdx.x % WAVE) == 0) {
+    sh[w][0] = a;
+    sh[w][1] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t ta = red_identity<OP>(), tb = red_identity<OP>();
+    for (int i = 0; i < BLOCK / WAVE; ++i) {
+      ta = red_combine<OP>(ta, sh[i][0]);
+      tb = red_combine<OP>(tb, sh[i][1]);
+    }
+    done(ta, tb);
+    __hip_atomic_store(s.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Value of slot j (runtime) of a lane packed at runtime width bw (bw <= W).
+template <typename UT>
+__device__ __forceinline__ UT extract_runtime(const UT *__restrict in_lane, int bw, int j) {
+  constexpr int W = lanes<UT>::W;
+  constexpr int NL = lanes<UT>::N;
+  if (bw == 0) return UT(0);
+  const int p = j * bw;
+  const int r = p / W;
+  const int s = p % W;
+  UT v = UT(in_lane[r * NL] >> s);
+  if (s + bw > W) v |= UT(in_lane[(r + 1) * NL] << (W - s));
+  if (bw < W) v &= UT((UT(1) << bw) - 1);
+  return v;
+}
+
+}  // namespace flsgpu_hip

@@ -1,0 +1,141 @@
+// FastLanes lane unpacking specialised per bit width for gfx950.
+//
+// Semantics follow the reference CPU unpacker (src/fls/unffor.cpp, e.g.
+// unffor_3bw_32ow_32crw_1uf :2982) and the device unpackers of
+// src/flsgpu/fls.cuh:43-618: the value in lane l, slot j of a vector packed at
+// width BW is bits [j*BW, (j+1)*BW) of the lane's word stream
+// in[l + N_LANES*r], least significant bit first.
+//
+// Design for CDNA4 (not a translation of the reference's unpacker family):
+// every bit width gets its own fully unrolled body, so word indices and shift
+// amounts are immediates. There is no runtime shift by W (the reference's
+// branchless unpackers rely on CUDA clamping `x >> 32` to 0, which AMD
+// hardware does not do: v_lshrrev_b32 masks the count to 5 bits). A value
+// straddling two words becomes one v_alignbit_b32 (or a 64-bit funnel).
+// A lane may be split over S threads (S = 1, 2, 4); each handles a
+// contiguous, compile-time slot range [J0, J0 + NJ) and loads only the words
+// that range touches.
+#pragma once
+
+#include <utility>
+
+#include "common.hpp"
+
+namespace flsgpu_hip {
+
+template <int... Js, class F>
+__device__ __forceinline__ void static_for_impl(std::integer_sequence<int, Js...>, F &&f) {
+  (f(std::integral_constant<int, Js>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_impl(std::make_integer_sequence<int, N>{}, f);
+}
+
+// Calls emit(j, value) for slots J0..J0+NJ-1 of one lane. `in` points at the
+// lane's word 0 of the vector (packed + vector_offset + lane).
+template <typename UT, int BW, int J0, int NJ, class Emit>
+__device__ __forceinline__ void unpack_slots(const UT *__restrict in, Emit &&emit) {
+  constexpr int W = lanes<UT>::W;
+  constexpr int NL = lanes<UT>::N;
+  static_assert(BW >= 0 && BW <= W, "bit width out of range");
+  if constexpr (BW == 0) {
+    static_for<NJ>([&](auto jc) { emit(J0 + int(decltype(jc)::value), UT(0)); });
+  } else {
+    constexpr int R0 = (J0 * BW) / W;
+    constexpr int R1 = ((J0 + NJ) * BW - 1) / W;
+    constexpr int NR = R1 - R0 + 1;
+    constexpr UT MASK = BW == W ? UT(~UT(0)) : UT((UT(1) << BW) - 1);
+    UT w[NR];
+    static_for<NR>([&](auto rc) {
+      constexpr int r = decltype(rc)::value;
+      w[r] = in[(R0 + r) * NL];
+    });
+    static_for<NJ>([&](auto jc) {
+      constexpr int j = J0 + int(decltype(jc)::value);
+      constexpr int p = j * BW;
+      constexpr int r = p / W - R0;
+      constexpr int s = p % W;
+      UT v;
+      if constexpr (s + BW > W) {
+        v = UT(w[r] >> s) | UT(w[r + 1] << (W - s));
+      } else {
+        v = UT(w[r] >> s);
+      }
+      if constexpr (BW < W) v &= MASK;
+      emit(j, v);
+    });
+  }
+}
+
+// Dispatch on a wave-uniform bit width and slot part. Policy provides
+//   template <int BW, int J0, int NJ> __device__ void run();
+template <typename UT, int S, int BW, class Policy>
+__device__ __forceinline__ void dispatch_part(Policy &pol, int part) {
+  constexpr int W = lanes<UT>::W;
+  constexpr int NJ = W / S;
+  if constexpr (S == 1) {
+    pol.template run<BW, 0, NJ>();
+  } else if constexpr (S == 2) {
+    if (part == 0) pol.template run<BW, 0, NJ>();
+    else pol.template run<BW, NJ, NJ>();
+  } else {
+    static_assert(S == 4, "S must be 1, 2 or 4");
+    switch (part) {
+      case 0: pol.template run<BW, 0, NJ>(); break;
+      case 1: pol.template run<BW, NJ, NJ>(); break;
+      case 2: pol.template run<BW, 2 * NJ, NJ>(); break;
+      default: pol.template run<BW, 3 * NJ, NJ>(); break;
+    }
+  }
+}
+
+#define FLSGPU_CASE(B)                                                          \
+  case B:                                                                      \
+    if constexpr ((B) <= lanes<UT>::W) dispatch_part<UT, S, (B)>(pol, part);    \
+    break;
+#define FLSGPU_CASES_0_32(X)                                                   \
+  X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) \
+  X(15) X(16) X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27)  \
+  X(28) X(29) X(30) X(31) X(32)
+#define FLSGPU_CASES_33_64(X)                                                  \
+  X(33) X(34) X(35) X(36) X(37) X(38) X(39) X(40) X(41) X(42) X(43) X(44) X(45)  \
+  X(46) X(47) X(48) X(49) X(50) X(51) X(52) X(53) X(54) X(55) X(56) X(57) X(58)  \
+  X(59) X(60) X(61) X(62) X(63) X(64)
+
+// `bw` must be wave-uniform here (an SGPR value).
+template <typename UT, int S, class Policy>
+__device__ __forceinline__ bool dispatch_uniform(Policy &pol, int bw, int part) {
+  if constexpr (lanes<UT>::W == 32) {
+    switch (bw) {
+      FLSGPU_CASES_0_32(FLSGPU_CASE)
+      default: return false;
+    }
+  } else {
+    switch (bw) {
+      FLSGPU_CASES_0_32(FLSGPU_CASE)
+      FLSGPU_CASES_33_64(FLSGPU_CASE)
+      default: return false;
+    }
+  }
+  return true;
+}
+
+// Waterfall over the distinct bit widths present in the wavefront: each
+// distinct width runs its specialised body once with the matching lanes
+// active. With one width per wave (the common case) this is a single pass.
+// Returns false for lanes whose width exceeds W (nothing was emitted).
+template <typename UT, int S, class Policy>
+__device__ __forceinline__ bool dispatch_bw(Policy &pol, int bw, int part) {
+  bool ok = true;
+  for (;;) {
+    const int b = __builtin_amdgcn_readfirstlane(bw);
+    if (b == bw) {
+      ok = dispatch_uniform<UT, S>(pol, b, part);
+      break;
+    }
+  }
+  return ok;
+}
+
+}  // namespace flsgpu_hip

@@ -1,0 +1,692 @@
+/*
+ * Oracle (test infrastructure only): ALP / G-ALP / ALP-RD restated in C.
+ *
+ * Follows, value for value:
+ *   decode formula      src/flsgpu/alp.cuh:40-45, src/alp/falp.cpp (e.g. :95-130)
+ *   patch               src/alp/decoder.hpp:137-145
+ *   column decode       src/alp/alp-bindings.cu:208-255
+ *   G-ALP conversion    src/flsgpu/structs.cuh:231-306
+ *   sampler             src/alp/sampler.hpp:11-48
+ *   encode_value        src/alp/encoder.hpp:76-84
+ *   decode_value        src/alp/decoder.hpp:121-124
+ *   top-k search        src/alp/encoder.hpp:134-230
+ *   per-vector choice   src/alp/encoder.hpp:236-300
+ *   encode_simdized     src/alp/encoder.hpp:302-379
+ *   analyze_ffor        src/alp/encoder.hpp:104-115
+ *   column encode       src/alp/alp-bindings.cu:63-206
+ *   ALP-RD              src/alp/rd.hpp:12-180
+ *
+ * Known divergence, documented: at bit width == W the reference's generated
+ * CPU falp (falp.cpp:42198 for f32, :33293 for f64) multiplies the unsigned
+ * register by the fraction instead of the converted integer. This oracle
+ * follows the decode formula of the reference decoder API (alp.cuh:40-45)
+ * at every width; tests compare against the compiled reference only for
+ * widths < W.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+/* ---- tables: src/alp/constants.hpp:49-55 (f32), :76-144 (f64) ---------- */
+const float orc_alp_frac_f32[11] = {1.0,    0.1,     0.01,     0.001,     0.0001,      0.00001,
+                                    0.000001, 0.0000001, 0.00000001, 0.000000001, 0.0000000001};
+const float orc_alp_exp_f32[11] = {1.0,       10.0,       100.0,       1000.0,
+                                   10000.0,   100000.0,   1000000.0,   10000000.0,
+                                   100000000.0, 1000000000.0, 10000000000.0};
+const int32_t orc_alp_fact_f32[10] = {1,      10,      100,      1000,      10000,
+                                      100000, 1000000, 10000000, 100000000, 1000000000};
+const double orc_alp_frac_f64[21] = {1.0,
+                                     0.1,
+                                     0.01,
+                                     0.001,
+                                     0.0001,
+                                     0.00001,
+                                     0.000001,
+                                     0.0000001,
+                                     0.00000001,
+                                     0.000000001,
+                                     0.0000000001,
+                                     0.00000000001,
+                                     0.000000000001,
+                                     0.0000000000001,
+                                     0.00000000000001,
+                                     0.000000000000001,
+                                     0.0000000000000001,
+                                     0.00000000000000001,
+                                     0.000000000000000001,
+                                     0.0000000000000000001,
+                                     0.00000000000000000001};
+const double orc_alp_exp_f64[24] = {1.0,
+                                    10.0,
+                                    100.0,
+                                    1000.0,
+                                    10000.0,
+                                    100000.0,
+                                    1000000.0,
+                                    10000000.0,
+                                    100000000.0,
+                                    1000000000.0,
+                                    10000000000.0,
+                                    100000000000.0,
+                                    1000000000000.0,
+                                    10000000000000.0,
+                                    100000000000000.0,
+                                    1000000000000000.0,
+                                    10000000000000000.0,
+                                    100000000000000000.0,
+                                    1000000000000000000.0,
+                                    10000000000000000000.0,
+                                    100000000000000000000.0,
+                                    1000000000000000000000.0,
+                                    10000000000000000000000.0,
+                                    100000000000000000000000.0};
+const int64_t orc_alp_fact_f64[19] = {1LL,
+                                      10LL,
+                                      100LL,
+                                      1000LL,
+                                      10000LL,
+                                      100000LL,
+                                      1000000LL,
+                                      10000000LL,
+                                      100000000LL,
+                                      1000000000LL,
+                                      10000000000LL,
+                                      100000000000LL,
+                                      1000000000000LL,
+                                      10000000000000LL,
+                                      100000000000000LL,
+                                      1000000000000000LL,
+                                      10000000000000000LL,
+                                      100000000000000000LL,
+                                      1000000000000000000LL};
+
+#define ENCODING_UPPER_LIMIT 9223372036854774784.0
+#define ENCODING_LOWER_LIMIT -9223372036854774784.0
+#define MAGIC_F32 12582912.0f
+#define MAGIC_F64 6755399441055744.0 /* 0x0018000000000000 as a double */
+#define EXCEPTION_POSITION_SIZE 16
+#define RD_SIZE_THRESHOLD_F32 (22 * 32)
+#define RD_SIZE_THRESHOLD_F64 (48 * 32)
+#define MAX_K 5
+
+/* ---- decode ------------------------------------------------------------- */
+void orc_alp_decode_vec_f32(const uint32_t *in, float *out, unsigned bw, uint32_t base,
+                            unsigned fac, unsigned exp) {
+  uint32_t u[1024];
+  orc_fls_unpack_u32(in, u, bw);
+  const uint32_t factor = (uint32_t)orc_alp_fact_f32[fac];
+  const float frac = orc_alp_frac_f32[exp];
+  for (int i = 0; i < 1024; ++i) {
+    const int32_t s = (int32_t)((uint32_t)(u[i] + base) * factor);
+    out[i] = (float)s * frac;
+  }
+}
+
+void orc_alp_decode_vec_f64(const uint64_t *in, double *out, unsigned bw, uint64_t base,
+                            unsigned fac, unsigned exp) {
+  uint64_t u[1024];
+  orc_fls_unpack_u64(in, u, bw);
+  const uint64_t factor = (uint64_t)orc_alp_fact_f64[fac];
+  const double frac = orc_alp_frac_f64[exp];
+  for (int i = 0; i < 1024; ++i) {
+    const int64_t s = (int64_t)((uint64_t)(u[i] + base) * factor);
+    out[i] = (double)s * frac;
+  }
+}
+
+#define DEFINE_COLUMN_DECODE(SUF, T, UT, N_LANES)                                                 \
+  void orc_alp_column_decode_##SUF(                                                                \
+      size_t n_vecs, const UT *packed, const uint8_t *bit_widths, const uint64_t *vector_offsets,  \
+      const UT *bases, const uint8_t *factor_indices, const uint8_t *fraction_indices,             \
+      const uint64_t *exceptions_offsets, const T *exceptions, const uint16_t *positions,          \
+      const uint16_t *counts, const uint16_t *offsets_counts, T *out) {                            \
+    for (size_t v = 0; v < n_vecs; ++v) {                                                          \
+      T *o = out + v * 1024;                                                                       \
+      orc_alp_decode_vec_##SUF(packed + vector_offsets[v], o, bit_widths[v], bases[v],             \
+                               factor_indices[v], fraction_indices[v]);                            \
+      unsigned count = 0;                                                                          \
+      if (counts) {                                                                                \
+        count = counts[v];                                                                         \
+      } else {                                                                                     \
+        for (unsigned l = 0; l < (N_LANES); ++l) count += offsets_counts[v * (N_LANES) + l] >> 10; \
+        count &= 0xFFFFu; /* the reference accumulates into a uint16_t */                          \
+      }                                                                                            \
+      const T *e = exceptions + exceptions_offsets[v];                                             \
+      const uint16_t *p = positions + exceptions_offsets[v];                                       \
+      for (unsigned k = 0; k < count; ++k) o[p[k]] = e[k];                                         \
+    }                                                                                              \
+  }
+
+DEFINE_COLUMN_DECODE(f32, float, uint32_t, 32)
+DEFINE_COLUMN_DECODE(f64, double, uint64_t, 16)
+
+struct mt_alp_job {
+  size_t begin, end;
+  const uint64_t *packed;
+  const uint8_t *bit_widths;
+  const uint64_t *vector_offsets;
+  const uint64_t *bases;
+  const uint8_t *fi, *ri;
+  const uint64_t *eo;
+  const double *exc;
+  const uint16_t *pos, *cnt;
+  double *out;
+};
+
+static void *mt_alp_worker(void *arg) {
+  struct mt_alp_job *j = (struct mt_alp_job *)arg;
+  for (size_t v = j->begin; v < j->end; ++v) {
+    double *o = j->out + v * 1024;
+    orc_alp_decode_vec_f64(j->packed + j->vector_offsets[v], o, j->bit_widths[v], j->bases[v],
+                           j->fi[v], j->ri[v]);
+    for (unsigned k = 0; k < j->cnt[v]; ++k) o[j->pos[j->eo[v] + k]] = j->exc[j->eo[v] + k];
+  }
+  return NULL;
+}
+
+void orc_alp_column_decode_f64_mt(size_t n_vecs, const uint64_t *packed, const uint8_t *bit_widths,
+                                  const uint64_t *vector_offsets, const uint64_t *bases,
+                                  const uint8_t *factor_indices, const uint8_t *fraction_indices,
+                                  const uint64_t *exceptions_offsets, const double *exceptions,
+                                  const uint16_t *positions, const uint16_t *counts, double *out,
+                                  int n_threads) {
+  if (n_threads < 1) n_threads = 1;
+  if (n_threads > 256) n_threads = 256;
+  pthread_t th[256];
+  struct mt_alp_job jobs[256];
+  for (int t = 0; t < n_threads; ++t) {
+    struct mt_alp_job *j = &jobs[t];
+    j->begin = n_vecs * (size_t)t / (size_t)n_threads;
+    j->end = n_vecs * (size_t)(t + 1) / (size_t)n_threads;
+    j->packed = packed;
+    j->bit_widths = bit_widths;
+    j->vector_offsets = vector_offsets;
+    j->bases = bases;
+    j->fi = factor_indices;
+    j->ri = fraction_indices;
+    j->eo = exceptions_offsets;
+    j->exc = exceptions;
+    j->pos = positions;
+    j->cnt = counts;
+    j->out = out;
+    pthread_create(&th[t], NULL, mt_alp_worker, j);
+  }
+  for (int t = 0; t < n_threads; ++t) pthread_join(th[t], NULL);
+}
+
+/* ---- G-ALP conversion --------------------------------------------------- */
+#define DEFINE_GALP(SUF, T, N_LANES, VPL)                                                         \
+  void orc_galp_convert_##SUF(size_t n_vecs, const T *exceptions, const uint16_t *positions,       \
+                              const uint16_t *counts, T *out_exceptions, uint16_t *out_positions,  \
+                              uint16_t *out_offsets_counts) {                                      \
+    T *lane_exc = (T *)malloc(sizeof(T) * 1024);                                                   \
+    uint16_t *lane_pos = (uint16_t *)malloc(sizeof(uint16_t) * 1024);                              \
+    uint16_t lane_counts[N_LANES];                                                                 \
+    for (size_t v = 0; v < n_vecs; ++v) {                                                          \
+      const unsigned c = counts[v];                                                                \
+      memset(lane_counts, 0, sizeof(lane_counts));                                                 \
+      for (unsigned k = 0; k < c; ++k) {                                                           \
+        const unsigned lane = positions[k] % (N_LANES);                                            \
+        lane_exc[lane * (VPL) + lane_counts[lane]] = exceptions[k];                                \
+        lane_pos[lane * (VPL) + lane_counts[lane]] = positions[k];                                 \
+        ++lane_counts[lane];                                                                       \
+      }                                                                                            \
+      unsigned w = 0;                                                                              \
+      for (unsigned lane = 0; lane < (N_LANES); ++lane) {                                          \
+        for (unsigned k = 0; k < lane_counts[lane]; ++k, ++w) {                                    \
+          out_exceptions[w] = lane_exc[lane * (VPL) + k];                                          \
+          out_positions[w] = lane_pos[lane * (VPL) + k];                                           \
+        }                                                                                          \
+        out_offsets_counts[lane] =                                                                 \
+            (uint16_t)(((unsigned)lane_counts[lane] << 10) | (w - lane_counts[lane]));             \
+      }                                                                                            \
+      exceptions += c;                                                                             \
+      positions += c;                                                                              \
+      out_exceptions += c;                                                                         \
+      out_positions += c;                                                                          \
+      out_offsets_counts += (N_LANES);                                                             \
+    }                                                                                              \
+    free(lane_exc);                                                                                \
+    free(lane_pos);                                                                                \
+  }
+
+DEFINE_GALP(f32, float, 32, 32)
+DEFINE_GALP(f64, double, 16, 64)
+
+/* ---- sampler ------------------------------------------------------------ */
+#define ROWGROUP_SIZE 102400u
+#define ROWGROUP_SAMPLES_JUMP 12u /* (102400 / 8) / 1024 */
+
+#define DEFINE_SAMPLER(SUF, T)                                                                     \
+  size_t orc_alp_sample_##SUF(const T *data, size_t offset, size_t n, T *sample) {                \
+    const size_t left = n - offset;                                                                \
+    const size_t portion = left < ROWGROUP_SIZE ? left : ROWGROUP_SIZE;                            \
+    const size_t avail = (size_t)ceil((double)portion / 1024.0);                                   \
+    size_t sample_idx = 0, data_idx = offset;                                                      \
+    for (size_t vi = 0; vi < avail; ++vi) {                                                        \
+      const size_t rem = n - data_idx;                                                             \
+      const size_t cur = rem < 1024 ? rem : 1024;                                                  \
+      if (vi % ROWGROUP_SAMPLES_JUMP != 0) {                                                       \
+        data_idx += cur;                                                                           \
+        continue;                                                                                  \
+      }                                                                                            \
+      int32_t inc = (int32_t)ceil((double)cur / 32.0);                                             \
+      if (inc < 1) inc = 1;                                                                        \
+      if (cur < 32 && sample_idx != 0) {                                                           \
+        data_idx += cur;                                                                           \
+        continue;                                                                                  \
+      }                                                                                            \
+      for (size_t i = 0; i < cur; i += (size_t)inc) sample[sample_idx++] = data[data_idx + i];    \
+      data_idx += cur;                                                                             \
+    }                                                                                              \
+    return sample_idx;                                                                             \
+  }
+
+DEFINE_SAMPLER(f32, float)
+DEFINE_SAMPLER(f64, double)
+
+/* ---- scalar encode / decode of one value -------------------------------- */
+static int impossible_f64(double n) {
+  return !isfinite(n) || isnan(n) || n > ENCODING_UPPER_LIMIT || n < ENCODING_LOWER_LIMIT ||
+         (n == 0.0 && signbit(n));
+}
+static int impossible_f32(float n) {
+  return !isfinite(n) || isnan(n) || (double)n > ENCODING_UPPER_LIMIT ||
+         (double)n < ENCODING_LOWER_LIMIT || (n == 0.0f && signbit(n));
+}
+
+int64_t orc_alp_encode_value_f64(double v, unsigned fac, unsigned exp, int safe) {
+  double t = v * orc_alp_exp_f64[exp] * orc_alp_frac_f64[fac];
+  if (safe && impossible_f64(t)) return (int64_t)ENCODING_UPPER_LIMIT;
+  t = t + MAGIC_F64 - MAGIC_F64;
+  return (int64_t)t;
+}
+
+int32_t orc_alp_encode_value_f32(float v, unsigned fac, unsigned exp, int safe) {
+  float t = v * orc_alp_exp_f32[exp] * orc_alp_frac_f32[fac];
+  /* The reference returns ENCODING_UPPER_LIMIT converted to int32_t, an
+   * out-of-range conversion; g++ folds it to INT32_MAX. The value only ever
+   * feeds a round-trip check that fails either way. */
+  if (safe && impossible_f32(t)) return INT32_MAX;
+  t = t + MAGIC_F32 - MAGIC_F32;
+  return (int32_t)t;
+}
+
+static inline double decode_value_f64(int64_t e, unsigned fac, unsigned exp) {
+  const int64_t s = (int64_t)((uint64_t)e * (uint64_t)orc_alp_fact_f64[fac]);
+  return (double)s * orc_alp_frac_f64[exp];
+}
+static inline float decode_value_f32(int32_t e, unsigned fac, unsigned exp) {
+  const int32_t s = (int32_t)((uint32_t)e * (uint32_t)orc_alp_fact_f32[fac]);
+  return (float)s * orc_alp_frac_f32[exp];
+}
+
+static inline unsigned count_bits_u64(uint64_t x) { return x == 0 ? 0 : 64 - __builtin_clzll(x); }
+static inline unsigned count_bits_u32(uint32_t x) { return x == 0 ? 0 : 32 - __builtin_clz(x); }
+
+/* ---- search ------------------------------------------------------------- */
+#define DEFINE_SEARCH(SUF, T, ST, UT, MAXE, EXCSZ, THRESH, CB)                                    \
+  static uint64_t search_one_##SUF(const T *smp, uint64_t samples_size, uint64_t worst,            \
+                                   uint8_t *out_exp, uint8_t *out_fac) {                           \
+    uint8_t found_fac = 0, found_exp = 0;                                                          \
+    uint64_t sample_est = worst;                                                                   \
+    for (int exp_ref = (MAXE); exp_ref >= 0; --exp_ref) {                                          \
+      for (int fac = exp_ref; fac >= 0; --fac) {                                                   \
+        unsigned exc = 0, nonexc = 0;                                                              \
+        ST mx = (ST)((UT)1 << (sizeof(UT) * 8 - 1)), mn = (ST)(((UT)1 << (sizeof(UT) * 8 - 1)) - 1); \
+        for (uint64_t i = 0; i < samples_size; ++i) {                                              \
+          const T a = smp[i];                                                                      \
+          const ST e = orc_alp_encode_value_##SUF(a, (unsigned)fac, (unsigned)exp_ref, 1);         \
+          const T d = decode_value_##SUF(e, (unsigned)fac, (unsigned)exp_ref);                     \
+          if (d == a) {                                                                            \
+            ++nonexc;                                                                              \
+            if (e > mx) mx = e;                                                                    \
+            if (e < mn) mn = e;                                                                    \
+          } else {                                                                                 \
+            ++exc;                                                                                 \
+          }                                                                                        \
+        }                                                                                          \
+        if (nonexc < 2) continue;                                                                  \
+        const uint64_t bits = CB((UT)((UT)mx - (UT)mn));                                           \
+        const uint64_t est =                                                                       \
+            samples_size * bits + (uint64_t)exc * ((EXCSZ) + EXCEPTION_POSITION_SIZE);             \
+        if (est < sample_est || (est == sample_est && found_exp < exp_ref) ||                      \
+            (est == sample_est && found_exp == exp_ref && found_fac < fac)) {                      \
+          sample_est = est;                                                                        \
+          found_exp = (uint8_t)exp_ref;                                                            \
+          found_fac = (uint8_t)fac;                                                                \
+        }                                                                                          \
+      }                                                                                            \
+    }                                                                                              \
+    *out_exp = found_exp;                                                                          \
+    *out_fac = found_fac;                                                                          \
+    return sample_est;                                                                             \
+  }                                                                                                \
+  uint64_t orc_alp_search_one_##SUF(const T *sample, size_t n_samples, uint8_t *exp,               \
+                                    uint8_t *fac) {                                                \
+    const uint64_t ss = n_samples < 32 ? n_samples : 32;                                           \
+    const uint64_t worst = ss * ((EXCSZ) + EXCEPTION_POSITION_SIZE) + ss * (EXCSZ);                \
+    return search_one_##SUF(sample, ss, worst, exp, fac);                                          \
+  }                                                                                                \
+  int orc_alp_top_k_##SUF(const T *sample, size_t n_samples, uint8_t *exps, uint8_t *facs) {      \
+    const uint64_t n_vec = (uint64_t)ceil((T)n_samples / (T)32);                                   \
+    const uint64_t ss = n_samples < 32 ? n_samples : 32;                                           \
+    const uint64_t worst = ss * ((EXCSZ) + EXCEPTION_POSITION_SIZE) + ss * (EXCSZ);                \
+    uint64_t best = worst;                                                                         \
+    int counts[(MAXE) + 1][(MAXE) + 1];                                                            \
+    memset(counts, 0, sizeof(counts));                                                             \
+    uint64_t off = 0;                                                                              \
+    for (uint64_t s = 0; s < n_vec; ++s) {                                                         \
+      uint8_t e, f;                                                                                \
+      const uint64_t est = search_one_##SUF(sample + off, ss, worst, &e, &f);                      \
+      if (est < best) best = est;                                                                  \
+      counts[e][f]++;                                                                              \
+      off += ss;                                                                                   \
+    }                                                                                              \
+    if (best >= (THRESH)) return 0;                                                                \
+    /* order: count desc, exponent desc, factor desc (encoder.hpp:123-128) */                     \
+    int k = 0;                                                                                     \
+    int taken[(MAXE) + 1][(MAXE) + 1];                                                             \
+    memset(taken, 0, sizeof(taken));                                                               \
+    while (k < MAX_K) {                                                                            \
+      int be = -1, bf = -1, bc = 0;                                                                \
+      for (int e = (MAXE); e >= 0; --e)                                                            \
+        for (int f = (MAXE); f >= 0; --f)                                                          \
+          if (counts[e][f] > bc && !taken[e][f]) {                                                 \
+            bc = counts[e][f];                                                                     \
+            be = e;                                                                                \
+            bf = f;                                                                                \
+          }                                                                                        \
+      if (be < 0) break;                                                                           \
+      taken[be][bf] = 1;                                                                           \
+      exps[k] = (uint8_t)be;                                                                       \
+      facs[k] = (uint8_t)bf;                                                                       \
+      ++k;                                                                                         \
+    }                                                                                              \
+    return k;                                                                                      \
+  }                                                                                                \
+  void orc_alp_best_combination_##SUF(const uint8_t *exps, const uint8_t *facs, int k,            \
+                                      const T *vec, uint8_t *exp, uint8_t *fac) {                  \
+    uint8_t fe = 0, ff = 0;                                                                        \
+    uint64_t best = 0;                                                                             \
+    unsigned worse = 0;                                                                            \
+    for (int c = 0; c < k; ++c) {                                                                  \
+      const unsigned e_idx = exps[c], f_idx = facs[c];                                             \
+      unsigned exc = 0;                                                                            \
+      ST mx = (ST)((UT)1 << (sizeof(UT) * 8 - 1)), mn = (ST)(((UT)1 << (sizeof(UT) * 8 - 1)) - 1); \
+      for (unsigned i = 0; i < 1024; i += 32) {                                                    \
+        const T a = vec[i];                                                                        \
+        const ST en = orc_alp_encode_value_##SUF(a, f_idx, e_idx, 1);                              \
+        const T d = decode_value_##SUF(en, f_idx, e_idx);                                          \
+        if (d == a) {                                                                              \
+          if (en > mx) mx = en;                                                                    \
+          if (en < mn) mn = en;                                                                    \
+        } else {                                                                                   \
+          ++exc;                                                                                   \
+        }                                                                                          \
+      }                                                                                            \
+      const uint64_t est = 32u * (uint64_t)CB((UT)((UT)mx - (UT)mn)) +                             \
+                           (uint64_t)exc * ((EXCSZ) + EXCEPTION_POSITION_SIZE);                    \
+      if (c == 0) {                                                                                \
+        best = est;                                                                                \
+        ff = (uint8_t)f_idx;                                                                       \
+        fe = (uint8_t)e_idx;                                                                       \
+        continue;                                                                                  \
+      }                                                                                            \
+      if (est >= best) {                                                                           \
+        if (++worse == 2) break;                                                                   \
+        continue;                                                                                  \
+      }                                                                                            \
+      best = est;                                                                                  \
+      ff = (uint8_t)f_idx;                                                                         \
+      fe = (uint8_t)e_idx;                                                                         \
+      worse = 0;                                                                                   \
+    }                                                                                              \
+    *exp = fe;                                                                                     \
+    *fac = ff;                                                                                     \
+  }
+
+DEFINE_SEARCH(f32, float, int32_t, uint32_t, 10, 32, RD_SIZE_THRESHOLD_F32, count_bits_u32)
+DEFINE_SEARCH(f64, double, int64_t, uint64_t, 18, 64, RD_SIZE_THRESHOLD_F64, count_bits_u64)
+
+/* ---- encode one vector -------------------------------------------------- */
+#define DEFINE_ENCODE_VEC(SUF, T, ST, UT, SIGNMASK, EXPMASK, NEGZERO)                              \
+  unsigned orc_alp_encode_vec_##SUF(const T *vec, unsigned fac, unsigned exp, ST *encoded,         \
+                                    T *exceptions, uint16_t *positions) {                          \
+    T wo[1024];                                                                                    \
+    uint16_t idx[1024];                                                                            \
+    for (unsigned i = 0; i < 1024; ++i) {                                                          \
+      UT bits;                                                                                     \
+      memcpy(&bits, &vec[i], sizeof(bits));                                                        \
+      const int special = ((bits & (SIGNMASK)) >= (EXPMASK)) || bits == (NEGZERO);                 \
+      wo[i] = special ? (T)ENCODING_UPPER_LIMIT : vec[i];                                          \
+    }                                                                                              \
+    unsigned n_exc = 0;                                                                            \
+    for (unsigned i = 0; i < 1024; ++i) {                                                          \
+      encoded[i] = orc_alp_encode_value_##SUF(wo[i], fac, exp, 0);                                 \
+      const T d = decode_value_##SUF(encoded[i], fac, exp);                                        \
+      idx[n_exc] = (uint16_t)i;                                                                    \
+      n_exc += (d != wo[i]);                                                                       \
+    }                                                                                              \
+    /* encoder.hpp:352-358: value of the first index i with INDEX_ARR[i] != i;                  \
+     * for sorted exception positions this is the first non-exception slot,                     \
+     * except when slots 0..1022 are all exceptions (then INDEX_ARR[1023] was                   \
+     * last written with 1023 and the value stays 0). */                                         \
+    ST fill = 0;                                                                                   \
+    for (unsigned i = 0; i < 1024; ++i) {                                                          \
+      const unsigned at = i < n_exc ? idx[i] : (i == n_exc ? 1023u : i);                           \
+      if (i != at) {                                                                               \
+        fill = encoded[i];                                                                         \
+        break;                                                                                     \
+      }                                                                                            \
+    }                                                                                              \
+    for (unsigned k = 0; k < n_exc; ++k) {                                                         \
+      const unsigned i = idx[k];                                                                   \
+      encoded[i] = fill;                                                                           \
+      exceptions[k] = vec[i];                                                                      \
+      positions[k] = (uint16_t)i;                                                                  \
+    }                                                                                              \
+    return n_exc;                                                                                  \
+  }                                                                                                \
+  void orc_alp_analyze_ffor_##SUF(const ST *encoded, uint8_t *bw, ST *base) {                      \
+    ST mn = (ST)(((UT)1 << (sizeof(UT) * 8 - 1)) - 1), mx = (ST)((UT)1 << (sizeof(UT) * 8 - 1));   \
+    for (unsigned i = 0; i < 1024; ++i) {                                                          \
+      if (encoded[i] < mn) mn = encoded[i];                                                        \
+      if (encoded[i] > mx) mx = encoded[i];                                                        \
+    }                                                                                              \
+    const UT delta = (UT)((UT)mx - (UT)mn);                                                        \
+    *bw = (uint8_t)(sizeof(UT) == 8 ? count_bits_u64((uint64_t)delta)                              \
+                                    : count_bits_u32((uint32_t)delta));                            \
+    *base = mn;                                                                                    \
+  }
+
+DEFINE_ENCODE_VEC(f32, float, int32_t, uint32_t, 0x7FFFFFFFu, 0x7F800000u, 0x80000000u)
+DEFINE_ENCODE_VEC(f64, double, int64_t, uint64_t, 0x7FFFFFFFFFFFFFFFull, 0x7FF0000000000000ull,
+                  0x8000000000000000ull)
+
+/* ---- column encode ------------------------------------------------------ */
+#define DEFINE_COLUMN_ENCODE(SUF, T, ST, UT, N_LANES)                                              \
+  int orc_alp_column_encode_##SUF(const T *in, size_t n_vecs, UT *packed, uint8_t *bit_widths,     \
+                                  uint64_t *vector_offsets, UT *bases, uint8_t *factor_indices,    \
+                                  uint8_t *fraction_indices, uint64_t *exceptions_offsets,         \
+                                  T *exceptions, uint16_t *positions, uint16_t *counts,            \
+                                  uint64_t *sizes) {                                               \
+    const size_t n = n_vecs * 1024;                                                                \
+    T *sample = (T *)malloc(sizeof(T) * (n < ROWGROUP_SIZE ? n : ROWGROUP_SIZE) + 64);             \
+    const size_t ns = orc_alp_sample_##SUF(in, 0, n, sample);                                      \
+    uint8_t exps[MAX_K], facs[MAX_K];                                                              \
+    const int k = orc_alp_top_k_##SUF(sample, ns, exps, facs);                                     \
+    free(sample);                                                                                  \
+    if (k == 0) return 1;                                                                          \
+    ST enc[1024];                                                                                  \
+    uint64_t w_off = 0, e_off = 0, vec_bytes = 0;                                                  \
+    for (size_t v = 0; v < n_vecs; ++v) {                                                          \
+      const T *vec = in + v * 1024;                                                                \
+      uint8_t e = exps[0], f = facs[0];                                                            \
+      if (k > 1) orc_alp_best_combination_##SUF(exps, facs, k, vec, &e, &f);                       \
+      const unsigned c =                                                                           \
+          orc_alp_encode_vec_##SUF(vec, f, e, enc, exceptions + e_off, positions + e_off);         \
+      uint8_t bw;                                                                                  \
+      ST base;                                                                                     \
+      orc_alp_analyze_ffor_##SUF(enc, &bw, &base);                                                 \
+      orc_fls_ffor_u##SUF##_w((const UT *)enc, packed + w_off, bw, (UT)base);                      \
+      bit_widths[v] = bw;                                                                          \
+      vector_offsets[v] = w_off;                                                                   \
+      bases[v] = (UT)base;                                                                         \
+      factor_indices[v] = f;                                                                       \
+      fraction_indices[v] = e;                                                                     \
+      exceptions_offsets[v] = e_off;                                                               \
+      counts[v] = (uint16_t)c;                                                                     \
+      w_off += (uint64_t)bw * (N_LANES);                                                           \
+      e_off += c;                                                                                  \
+      vec_bytes += (uint64_t)bw * (N_LANES) * sizeof(T) + (uint64_t)c * (sizeof(T) + 2);           \
+    }                                                                                              \
+    sizes[0] = w_off;                                                                              \
+    sizes[1] = e_off;                                                                              \
+    sizes[2] = vec_bytes + n_vecs * (3 + sizeof(T));                                               \
+    sizes[3] = vec_bytes + n_vecs * (3 + sizeof(T) + 2 * (N_LANES));                               \
+    return 0;                                                                                      \
+  }
+
+#define orc_fls_ffor_uf32_w orc_fls_ffor_u32
+#define orc_fls_ffor_uf64_w orc_fls_ffor_u64
+DEFINE_COLUMN_ENCODE(f32, float, int32_t, uint32_t, 32)
+DEFINE_COLUMN_ENCODE(f64, double, int64_t, uint64_t, 16)
+
+/* ---- ALP-RD --------------------------------------------------------------- */
+struct lp_count {
+  uint32_t value;
+  int32_t count;
+};
+
+static int cmp_u32(const void *a, const void *b) {
+  const uint32_t x = *(const uint32_t *)a, y = *(const uint32_t *)b;
+  return (x > y) - (x < y);
+}
+static int cmp_lp(const void *a, const void *b) {
+  const struct lp_count *x = (const struct lp_count *)a, *y = (const struct lp_count *)b;
+  if (x->count != y->count) return x->count > y->count ? -1 : 1;
+  return (x->value > y->value) - (x->value < y->value);
+}
+
+/* rd.hpp:27-81: counts of left parts, sorted by count; returns the estimated
+ * bits per value and (when dict != NULL) the dictionary. */
+static double rd_build(const uint32_t *lefts_sorted_scratch, size_t n, unsigned rbw,
+                       uint16_t *dict, unsigned *dict_size, unsigned *left_bw) {
+  struct lp_count *lp = (struct lp_count *)malloc(sizeof(struct lp_count) * (n + 1));
+  size_t n_unique = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (n_unique > 0 && lp[n_unique - 1].value == lefts_sorted_scratch[i]) {
+      lp[n_unique - 1].count++;
+    } else {
+      lp[n_unique].value = lefts_sorted_scratch[i];
+      lp[n_unique].count = 1;
+      ++n_unique;
+    }
+  }
+  qsort(lp, n_unique, sizeof(struct lp_count), cmp_lp);
+  uint32_t exc = 0;
+  for (size_t i = 8; i < n_unique; ++i) exc += (uint32_t)lp[i].count;
+  const unsigned ds = n_unique < 8 ? (unsigned)n_unique : 8u;
+  unsigned lbw = (unsigned)ceil(log2((double)ds));
+  if (lbw < 1) lbw = 1;
+  if (dict) {
+    for (unsigned i = 0; i < ds; ++i) dict[i] = (uint16_t)lp[i].value;
+    *dict_size = ds;
+    *left_bw = lbw;
+  }
+  free(lp);
+  /* rd.hpp:15-22: (right_bw + left_bw) as an integer, plus the exception bits
+   * per sampled value. */
+  const double exc_size = (double)(uint32_t)(exc * (16u + 16u));
+  return (double)(rbw + lbw) + exc_size / (double)n;
+}
+
+#define DEFINE_RD_DICT(SUF, T, UT, W)                                                              \
+  unsigned orc_alprd_find_dictionary_##SUF(const T *sample, size_t n, uint16_t *dict,              \
+                                           unsigned *dict_size, unsigned *left_bw) {               \
+    uint32_t *lefts = (uint32_t *)malloc(sizeof(uint32_t) * (n + 1));                              \
+    const UT *bits = (const UT *)sample;                                                           \
+    unsigned best_rbw = 0;                                                                         \
+    double best = 1.0e300;                                                                         \
+    for (unsigned i = 1; i <= 16; ++i) {                                                           \
+      const unsigned rbw = (W) - i;                                                                \
+      for (size_t s = 0; s < n; ++s) lefts[s] = (uint32_t)(bits[s] >> rbw);                       \
+      qsort(lefts, n, sizeof(uint32_t), cmp_u32);                                                  \
+      const double est = rd_build(lefts, n, rbw, NULL, NULL, NULL);                       \
+      if (est < best) {                                                                            \
+        best = est;                                                                                \
+        best_rbw = rbw;                                                                            \
+      }                                                                                            \
+    }                                                                                              \
+    for (size_t s = 0; s < n; ++s) lefts[s] = (uint32_t)(bits[s] >> best_rbw);                     \
+    qsort(lefts, n, sizeof(uint32_t), cmp_u32);                                                    \
+    rd_build(lefts, n, best_rbw, dict, dict_size, left_bw);                                                  \
+    free(lefts);                                                                                   \
+    return best_rbw;                                                                               \
+  }
+
+DEFINE_RD_DICT(f64, double, uint64_t, 64)
+DEFINE_RD_DICT(f32, float, uint32_t, 32)
+
+#define DEFINE_RD_COLUMN(SUF, T, UT, W)                                                            \
+  void orc_alprd_column_encode_##SUF(const T *in, size_t n_vecs, unsigned right_bw,                \
+                                     const uint16_t *dict, unsigned dict_size, unsigned left_bw,   \
+                                     UT *right_packed, uint16_t *left_packed,                      \
+                                     uint64_t *exceptions_offsets, uint16_t *exceptions,           \
+                                     uint16_t *positions, uint16_t *counts) {                      \
+    UT right[1024];                                                                                \
+    uint16_t left[1024];                                                                           \
+    uint64_t e_off = 0;                                                                            \
+    const UT rmask = right_bw >= (W) ? (UT)~(UT)0 : (UT)(((UT)1 << right_bw) - 1);                 \
+    for (size_t v = 0; v < n_vecs; ++v) {                                                          \
+      const UT *bits = (const UT *)(in + v * 1024);                                                \
+      unsigned c = 0;                                                                              \
+      for (unsigned i = 0; i < 1024; ++i) {                                                        \
+        right[i] = bits[i] & rmask;                                                                \
+        const uint16_t lp = (uint16_t)(bits[i] >> right_bw);                                       \
+        unsigned d = 0;                                                                            \
+        while (d < dict_size && dict[d] != lp) ++d;                                                \
+        if (d < dict_size) {                                                                       \
+          left[i] = (uint16_t)d;                                                                   \
+        } else {                                                                                   \
+          left[i] = 0;                                                                             \
+          exceptions[e_off + c] = lp;                                                              \
+          positions[e_off + c] = (uint16_t)i;                                                      \
+          ++c;                                                                                     \
+        }                                                                                          \
+      }                                                                                            \
+      orc_fls_pack_u##W(right, right_packed + v * (size_t)right_bw * (1024 / (W)), right_bw);     \
+      orc_fls_pack_u16(left, left_packed + v * (size_t)left_bw * 64, left_bw);                     \
+      exceptions_offsets[v] = e_off;                                                               \
+      counts[v] = (uint16_t)c;                                                                     \
+      e_off += c;                                                                                  \
+    }                                                                                              \
+  }                                                                                                \
+  void orc_alprd_column_decode_##SUF(size_t n_vecs, unsigned right_bw, const uint16_t *dict,      \
+                                     unsigned left_bw, const UT *right_packed,                     \
+                                     const uint16_t *left_packed,                                  \
+                                     const uint64_t *exceptions_offsets,                           \
+                                     const uint16_t *exceptions, const uint16_t *positions,        \
+                                     const uint16_t *counts, T *out) {                             \
+    UT right[1024];                                                                                \
+    uint16_t left[1024];                                                                           \
+    for (size_t v = 0; v < n_vecs; ++v) {                                                          \
+      orc_fls_unpack_u##W(right_packed + v * (size_t)right_bw * (1024 / (W)), right, right_bw);   \
+      orc_fls_unpack_u16(left_packed + v * (size_t)left_bw * 64, left, left_bw);                   \
+      UT *o = (UT *)(out + v * 1024);                                                              \
+      for (unsigned i = 0; i < 1024; ++i)                                                          \
+        o[i] = ((UT)dict[left[i]] << right_bw) | right[i];                                         \
+      const uint64_t eo = exceptions_offsets[v];                                                   \
+      for (unsigned k = 0; k < counts[v]; ++k) {                                                   \
+        const unsigned p = positions[eo + k];                                                      \
+        o[p] = ((UT)exceptions[eo + k] << right_bw) | right[p];                                    \
+      }                                                                                            \
+    }                                                                                              \
+  }
+
+DEFINE_RD_COLUMN(f64, double, uint64_t, 64)
+DEFINE_RD_COLUMN(f32, float, uint32_t, 32)
