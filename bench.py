@@ -79,12 +79,18 @@ def main():
         read = n_vecs * (bw * nl * width // 8 + 1 + 8 + width // 8)  # packed + bw + offset + base
         return read + nvals * width // 8
 
-    def run_step(events=None):
+    from flsgpu import autotune as AT
+    tuned = os.path.exists(AT.DEFAULT_TABLE) and args.variant == 0
+    if tuned:
+        AT.load_table()
+    variants = [AT.variant_for(f"ffor_u{w}", bw) if tuned else args.variant for w, bw, _ in cols]
+
+    def run_step(events=None, plain=False):
         for i, (width, bw, col) in enumerate(cols):
             out = out32 if width == 32 else out64
             if events is not None:
                 events[i][0].record(stream)
-            D.decompress(col, out, variant=args.variant, stream=stream)
+            D.decompress(col, out, variant=A.VARIANT_LANE if plain else variants[i], stream=stream)
             if events is not None:
                 events[i][1].record(stream)
 
@@ -109,6 +115,15 @@ def main():
 
     dec_bytes = sum(nvals * w // 8 for w, _, _ in cols)
     value = dec_bytes / (step_ms / 1e3) / 1e9
+
+    # the same sweep with the plain one-thread-per-lane variant, for comparison
+    run_step(plain=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_step(plain=True)
+    torch.cuda.synchronize()
+    plain_ms = (time.perf_counter() - t0) * 1e3 / args.steps
 
     per_width = []
     for i, (width, bw, _) in enumerate(cols):
@@ -148,13 +163,17 @@ def main():
                      "avg_algorithmic_bytes": int(avg_bytes)},
         "u32_sweep": {"avg_launch_us": round(avg_ms32 * 1e3, 2),
                       "hbm_frac": round(avg_bytes32 / (avg_ms32 / 1e3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "autotuned": tuned,
+        "plain_lane_variant": {"value": round(dec_bytes / (plain_ms / 1e3) / 1e9, 1),
+                               "ms_per_step": round(plain_ms, 3)},
+        "hbm_copy_GBps": round(copy_bandwidth(torch, dev, stream), 1),
+        "per_width": per_width,
     }
 
     del cols
     torch.cuda.empty_cache()
     if not args.no_extras:
         result["extras"] = extras(args, torch, A, D, E, dev, stream, out32)
-        result["per_width"] = per_width
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args)
     line = json.dumps(result)
@@ -162,6 +181,17 @@ def main():
     if args.out:
         with open(args.out, "w") as f:
             f.write(line + "\n")
+
+
+def copy_bandwidth(torch, dev, stream):
+    """Achievable HBM bandwidth for reference: a 2 GiB device-to-device copy
+    (read + write bytes per second)."""
+    a = torch.empty(1 << 28, dtype=torch.int64, device=dev)
+    b = torch.empty_like(a)
+    a.fill_(1)
+    ms = time_launches(torch, stream, lambda: b.copy_(a), 10)
+    del a, b
+    return 2 * (1 << 31) / ms / 1e6
 
 
 def time_launches(torch, stream, fn, reps):
@@ -202,7 +232,7 @@ def extras(args, torch, A, D, E, dev, stream, out32):
     r = torch.empty(2, dtype=torch.uint64, device=dev)
 
     def fsum():
-        D.query_filter_sum(cols[k[0] % 4], A.PRED_RANGE, 0, 2 ** 32 - 1, r, stream=stream)
+        D.query_filter_sum(cols[k[0] % 4], A.PRED_ALL, result=r, stream=stream)
         k[0] += 1
     ms = time_launches(torch, stream, fsum, 50)
     got_sum = int(r.cpu().numpy()[1])
@@ -228,6 +258,103 @@ def extras(args, torch, A, D, E, dev, stream, out32):
             del col, pk, bb, oo, ss
     res["filter_sum_sweep"] = fs
     torch.cuda.empty_cache()
+    res.update(alp_extras(torch, dev, stream))
+    res["c4_batch"] = batch_extras(torch, dev, stream)
+    return res
+
+
+def batch_extras(torch, dev, stream):
+    """96 columns, log-uniform lengths 2^12..2^24, a third each FFOR u32
+    (width 1..24), ALP double (G-ALP layout) and ALP-RD double: one plain
+    launch per column vs one persistent batched launch vs its hipGraph replay."""
+    import numpy as np
+    from flsgpu import batch as B
+    from flsgpu import workloads as WL
+    t0 = time.perf_counter()
+    specs = WL.mixed_batch(seed=4, n_columns=96)
+    gen_s = time.perf_counter() - t0
+    cols = [WL.to_device(s, dev) for s in specs]
+    outs = [B.output_for(c, dev) for c in cols]
+    plan = B.BatchPlan(cols, outs)
+    dec_bytes = sum(s.values.nbytes for s in specs)
+    reps = 10
+    t_plain = time_launches(torch, stream, lambda: B.decode_each(cols, outs, stream=stream), reps)
+    t_batch = time_launches(torch, stream, lambda: plan.run(stream), reps)
+    t_graph = time_launches(torch, stream, lambda: plan.graph_launch(stream), reps)
+    ok = True
+    for s, o in zip(specs, outs):
+        v = s.values.view(np.int32 if s.values.itemsize == 4 else np.int64)
+        ok &= bool(torch.equal(o.view(torch.int32 if s.values.itemsize == 4 else torch.int64),
+                               torch.from_numpy(v).to(dev)))
+    kinds = {}
+    for s in specs:
+        kinds[s.kind] = kinds.get(s.kind, 0) + 1
+    res = {"columns": len(specs), "kinds": kinds, "values": int(sum(len(s.values) for s in specs)),
+           "tasks": plan.n_tasks, "bit_exact": ok, "host_generate_encode_s": round(gen_s, 1),
+           "plain_per_column_us": round(t_plain * 1e3, 1),
+           "batched_us": round(t_batch * 1e3, 1), "graph_us": round(t_graph * 1e3, 1),
+           "plain_GBps": round(dec_bytes / t_plain / 1e6, 1),
+           "batched_GBps": round(dec_bytes / t_batch / 1e6, 1),
+           "graph_GBps": round(dec_bytes / t_graph / 1e6, 1),
+           "speedup_batched_vs_plain": round(t_plain / t_batch, 3),
+           "speedup_graph_vs_plain": round(t_plain / t_graph, 3)}
+    plan.close()
+    del cols, outs
+    torch.cuda.empty_cache()
+    return res
+
+
+def c3_values(np, width, rng):
+    """2^26 doubles k/100, k ~ U[-10^6, 10^6] (2 decimals in [-10^4, 10^4]), or
+    2^27 floats k/10, k ~ U[-10^4, 10^4]; in every 1024-vector 10 positions
+    (1%) replaced by full-precision uniform values."""
+    if width == 64:
+        n = 1 << 26
+        x = rng.integers(-1_000_000, 1_000_001, n) / 100.0
+        repl = rng.uniform(-1e4, 1e4, (n // 1024, 10))
+    else:
+        n = 1 << 27
+        x = rng.integers(-10_000, 10_001, n).astype(np.float32) / np.float32(10.0)
+        repl = rng.uniform(-1e3, 1e3, (n // 1024, 10)).astype(np.float32)
+    nv = n // 1024  # 10 distinct positions per vector: an odd stride is coprime with 1024
+    pos = (rng.integers(0, 1024, (nv, 1)) + (2 * rng.integers(0, 512, (nv, 1)) + 1)
+           * np.arange(10)[None, :]) % 1024
+    v = x.reshape(-1, 1024)
+    np.put_along_axis(v, pos, repl, axis=1)
+    return x
+
+
+def alp_extras(torch, dev, stream):
+    import numpy as np
+    from flsgpu import alp as AL
+    res = {}
+    rng = np.random.default_rng(33)
+    for width in (64, 32):
+        x = c3_values(np, width, rng)
+        t0 = time.perf_counter()
+        col = AL.alp_encode(x)
+        enc_s = time.perf_counter() - t0
+        galp = AL.galp_convert(col)
+        ref = torch.from_numpy(x.view(np.int64 if width == 64 else np.int32)).to(dev)
+        nbytes = x.nbytes
+        for name, c in (("alp", col), ("galp", galp)):
+            d = AL.DeviceAlpColumn.from_host(c, dev)
+            out = torch.empty(len(x), dtype=torch.float64 if width == 64 else torch.float32, device=dev)
+            ms = time_launches(torch, stream, lambda: AL.decompress(d, out, stream=stream), 20)
+            ok = bool(torch.equal(out.view(ref.dtype), ref))
+            read = (c.packed.nbytes + c.bit_widths.nbytes + c.vector_offsets.nbytes + c.bases.nbytes
+                    + c.factor_indices.nbytes * 2 + c.exceptions_offsets.nbytes + c.exceptions.nbytes
+                    + c.positions.nbytes + (c.counts.nbytes if c.offsets_counts is None else c.offsets_counts.nbytes))
+            res[f"c3_{name}_f{width}_2^{int(np.log2(len(x)))}"] = {
+                "us": round(ms * 1e3, 1), "decompressed_GBps": round(nbytes / ms / 1e6, 1),
+                "hbm_frac": round((read + nbytes) / ms / 1e6 / HBM_PEAK_GBS, 4), "bit_exact": ok,
+                "bits_per_value": round(float(col.bit_widths.mean()), 2),
+                "exceptions_per_vector": round(len(col.exceptions) / col.n_vecs, 2),
+                "compression_ratio": round(c.compression_ratio(extended=name == "galp"), 3),
+                "host_encode_s": round(enc_s, 2)}
+            del d, out
+        del ref
+        torch.cuda.empty_cache()
     return res
 
 

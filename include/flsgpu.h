@@ -126,16 +126,22 @@ enum flsgpu_predicate {
   FLSGPU_PRED_RANGE = 0,
   /* FOR offset (value - vector base) below half of the vector's range:
    * offset < 2^(bit_width-1); no row passes at bit_width 0. */
-  FLSGPU_PRED_OFFSET_LOWER_HALF = 1
+  FLSGPU_PRED_OFFSET_LOWER_HALF = 1,
+  /* every row: count and sum of the whole column */
+  FLSGPU_PRED_ALL = 2
 };
 
 /* ---- library ------------------------------------------------------------ */
 const char *flsgpu_last_error(void);
 /* Optional; the first call on a device initialises it lazily. */
 int flsgpu_init(int device);
-/* Loads the autotuner table (JSON written by flsgpu.autotune); NULL resets. */
-int flsgpu_set_tuning(uint32_t dtype_bits, uint32_t bit_width, int variant);
-int flsgpu_get_tuning(uint32_t dtype_bits, uint32_t bit_width);
+/* Autotuner table (replaces the SQL best-unpacker query,
+ * sql/select-best-unpacker-ffor.sql:1-7): the variant FLSGPU_VARIANT_AUTO
+ * resolves to for an encoding (enum flsgpu_encoding below) and bit width;
+ * bit_width 255 is the entry used when a call does not know the width.
+ * flsgpu.autotune measures the table and loads it through this call. */
+int flsgpu_set_tuning(uint32_t encoding, uint32_t bit_width, int variant);
+int flsgpu_get_tuning(uint32_t encoding, uint32_t bit_width);
 
 /* ---- device memory helpers (replace GPUArray / copy_to_device,
  *      src/flsgpu/host-utils.cuh:42-111, structs.cuh:103-229) --------------- */
@@ -228,6 +234,43 @@ int flsgpu_query_multi_column_alp_f32(const flsgpu_alp_column *cols, uint32_t n_
 int flsgpu_query_multi_column_alp_f64(const flsgpu_alp_column *cols, uint32_t n_cols,
                                       double value, uint8_t *d_found, int patcher, int variant,
                                       flsgpu_stream_t stream);
+
+/* ---- batched mixed-encoding decode (no reference counterpart: the
+ *      reference launches and synchronises once per column,
+ *      src/engine/kernels.cuh:94-112). One persistent launch decodes every
+ *      item from a device-side task queue; the queue re-arms itself, so the
+ *      launch can be replayed from a hipGraph. --------------------------- */
+enum flsgpu_encoding {
+  FLSGPU_ENC_BP_U32 = 0,    /* column: flsgpu_bp_column */
+  FLSGPU_ENC_FFOR_U32 = 1,  /* column: flsgpu_ffor_column */
+  FLSGPU_ENC_BP_U64 = 2,
+  FLSGPU_ENC_FFOR_U64 = 3,
+  FLSGPU_ENC_ALP_F32 = 4,   /* column: flsgpu_alp_column, standard layout */
+  FLSGPU_ENC_ALP_F64 = 5,
+  FLSGPU_ENC_ALP_EXT_F32 = 6, /* column: flsgpu_alp_column, G-ALP layout */
+  FLSGPU_ENC_ALP_EXT_F64 = 7,
+  FLSGPU_ENC_ALPRD_F32 = 8, /* column: flsgpu_alprd_column */
+  FLSGPU_ENC_ALPRD_F64 = 9
+};
+
+typedef struct flsgpu_batch_item {
+  int encoding;           /* enum flsgpu_encoding */
+  const void *column;     /* host pointer to the descriptor (read at plan creation) */
+  void *out;              /* device output of n_values elements */
+  int patcher;            /* ALP items: a patching patcher of the layout (none/dummy are rejected) */
+} flsgpu_batch_item;
+
+typedef struct flsgpu_batch_plan flsgpu_batch_plan;
+
+/* Copies the descriptors and a task list to the device (allocates). */
+int flsgpu_batch_plan_create(const flsgpu_batch_item *items, uint32_t n_items, int variant,
+                             flsgpu_batch_plan **plan);
+uint64_t flsgpu_batch_plan_tasks(const flsgpu_batch_plan *plan);
+/* One kernel launch for the whole batch. Launches of one plan must not overlap. */
+int flsgpu_batch_run(flsgpu_batch_plan *plan, flsgpu_stream_t stream);
+/* Captures flsgpu_batch_run into a hipGraph on first use, then replays it. */
+int flsgpu_batch_graph_launch(flsgpu_batch_plan *plan, flsgpu_stream_t stream);
+int flsgpu_batch_plan_destroy(flsgpu_batch_plan *plan);
 
 #ifdef __cplusplus
 }

@@ -66,8 +66,8 @@ static int make_alp_args(const flsgpu_alp_column *c, int layout, AlpArgs<T> *a) 
   return FLSGPU_OK;
 }
 
-static int alp_shape(int variant, int dtype_bits, int *S, bool *nt) {
-  if (variant == FLSGPU_VARIANT_AUTO) variant = flsgpu_get_tuning(dtype_bits, 255);
+static int alp_shape(int variant, int encoding, int *S, bool *nt) {
+  if (variant == FLSGPU_VARIANT_AUTO) variant = flsgpu_get_tuning(encoding, 255);
   switch (variant) {
     case FLSGPU_VARIANT_LANE: *S = 1; *nt = false; return FLSGPU_OK;
     case FLSGPU_VARIANT_LANE_NT: *S = 1; *nt = true; return FLSGPU_OK;
@@ -97,7 +97,9 @@ static int alp_decompress(const flsgpu_alp_column *c, T *out, int layout, int pa
   if ((rc = patch_mode(layout, patcher, &patch))) return rc;
   int S;
   bool nt;
-  if ((rc = alp_shape(variant, int(sizeof(T) * 8), &S, &nt))) return rc;
+  const int enc = sizeof(T) == 4 ? (layout == ALP_GALP ? FLSGPU_ENC_ALP_EXT_F32 : FLSGPU_ENC_ALP_F32)
+                                 : (layout == ALP_GALP ? FLSGPU_ENC_ALP_EXT_F64 : FLSGPU_ENC_ALP_F64);
+  if ((rc = alp_shape(variant, enc, &S, &nt))) return rc;
   if (S == 1) return nt ? decompress_s<T, 1, true>(a, out, layout, patch, st)
                         : decompress_s<T, 1, false>(a, out, layout, patch, st);
   if (S == 2) return nt ? decompress_s<T, 2, true>(a, out, layout, patch, st)

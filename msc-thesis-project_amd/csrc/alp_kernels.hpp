@@ -43,6 +43,11 @@ struct AlpDecompressPolicy {
       store<T, NT>(out + j * lanes<UT>::N, alp_value<T>(v, base, fact, frac));
     });
   }
+  __device__ __forceinline__ void run_generic(int bw, int j0, int nj) {
+    unpack_runtime<UT>(in, bw, j0, nj, [&](int j, UT v) {
+      store<T, NT>(out + j * lanes<UT>::N, alp_value<T>(v, base, fact, frac));
+    });
+  }
 };
 
 // Bit (j - J0) set when slot j decodes (before patching) to `target`.
@@ -59,6 +64,11 @@ struct AlpMatchPolicy {
   __device__ __forceinline__ void run() {
     unpack_slots<UT, BW, J0, NJ>(in, [&](int j, UT v) {
       mask |= uint64_t(alp_value<T>(v, base, fact, frac) == target) << (j - J0);
+    });
+  }
+  __device__ __forceinline__ void run_generic(int bw, int j0, int nj) {
+    unpack_runtime<UT>(in, bw, j0, nj, [&](int j, UT v) {
+      mask |= uint64_t(alp_value<T>(v, base, fact, frac) == target) << (j - j0);
     });
   }
 };
@@ -90,10 +100,10 @@ __device__ __forceinline__ void for_own_exceptions(const AlpArgs<T> &a, uint64_t
 }
 
 template <typename T, int S, bool NT, int LAYOUT, bool PATCH>
-__global__ __launch_bounds__(BLOCK) void k_alp_decompress(AlpArgs<T> a, T *__restrict out) {
+__device__ __forceinline__ void alp_decompress_unit(const AlpArgs<T> &a, T *__restrict out,
+                                                    uint64_t wave) {
   using UT = typename uint_of<T>::type;
   using L = lanes<UT>;
-  const uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE;
   const UnitMap<UT, S> u = map_unit<UT, S>(wave);
   if (u.vec >= a.ffor.n_vecs) return;
   T *vout = out + u.vec * VALUES_PER_VECTOR;
@@ -109,6 +119,12 @@ __global__ __launch_bounds__(BLOCK) void k_alp_decompress(AlpArgs<T> a, T *__res
     for_own_exceptions<T, LAYOUT>(a, u.vec, u.lane, j0, L::W / S,
                                   [&](unsigned pos, T e) { store<T, NT>(vout + pos, e); });
   }
+}
+
+template <typename T, int S, bool NT, int LAYOUT, bool PATCH>
+__global__ __launch_bounds__(BLOCK) void k_alp_decompress(AlpArgs<T> a, T *__restrict out) {
+  alp_decompress_unit<T, S, NT, LAYOUT, PATCH>(a, out,
+                                              (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE);
 }
 
 // Slot-match mask of one (vector, lane) after patching; exceptions replace

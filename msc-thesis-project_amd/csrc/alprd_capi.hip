@@ -32,7 +32,8 @@ static int alprd_decompress(const flsgpu_alprd_column *c, T *out, int variant, h
   a.exceptions = c->exceptions;
   a.positions = c->positions;
   a.counts = c->counts;
-  if (variant == FLSGPU_VARIANT_AUTO) variant = flsgpu_get_tuning(W, 255);
+  if (variant == FLSGPU_VARIANT_AUTO)
+    variant = flsgpu_get_tuning(W == 32 ? FLSGPU_ENC_ALPRD_F32 : FLSGPU_ENC_ALPRD_F64, 255);
   if (variant < FLSGPU_VARIANT_LANE || variant >= FLSGPU_VARIANT_COUNT)
     return set_error(FLSGPU_EUNSUPPORTED, "unknown variant %d", variant);
   // The non-temporal variants map to non-temporal stores; lane splitting is

@@ -39,36 +39,39 @@ struct RdPolicy {
   }
 };
 
+// right_bw is one value per column, so the dispatch is a scalar switch.
 template <typename UT, class Policy>
 __device__ __forceinline__ bool dispatch_right_bw(Policy &pol, int bw) {
   constexpr int W = lanes<UT>::W;
-  for (;;) {
-    const int b = __builtin_amdgcn_readfirstlane(bw);
-    if (b == bw) {
-      switch (b - (W - 16)) {
+  const int b = __builtin_amdgcn_readfirstlane(bw);
+  switch (b - (W - 16)) {
 #define FLSGPU_RD_CASE(K) \
   case K: pol.template run<W - 16 + K, 0, W>(); return true;
-        FLSGPU_RD_CASE(0) FLSGPU_RD_CASE(1) FLSGPU_RD_CASE(2) FLSGPU_RD_CASE(3)
-        FLSGPU_RD_CASE(4) FLSGPU_RD_CASE(5) FLSGPU_RD_CASE(6) FLSGPU_RD_CASE(7)
-        FLSGPU_RD_CASE(8) FLSGPU_RD_CASE(9) FLSGPU_RD_CASE(10) FLSGPU_RD_CASE(11)
-        FLSGPU_RD_CASE(12) FLSGPU_RD_CASE(13) FLSGPU_RD_CASE(14) FLSGPU_RD_CASE(15)
+    FLSGPU_RD_CASE(0) FLSGPU_RD_CASE(1) FLSGPU_RD_CASE(2) FLSGPU_RD_CASE(3)
+    FLSGPU_RD_CASE(4) FLSGPU_RD_CASE(5) FLSGPU_RD_CASE(6) FLSGPU_RD_CASE(7)
+    FLSGPU_RD_CASE(8) FLSGPU_RD_CASE(9) FLSGPU_RD_CASE(10) FLSGPU_RD_CASE(11)
+    FLSGPU_RD_CASE(12) FLSGPU_RD_CASE(13) FLSGPU_RD_CASE(14) FLSGPU_RD_CASE(15)
 #undef FLSGPU_RD_CASE
-        default: return false;
-      }
-    }
+    default: return false;
   }
 }
 
+// The block fills the pre-shifted dictionary in LDS before any wave decodes.
+template <typename T>
+__device__ __forceinline__ void alprd_fill_dictionary(const AlprdArgs<T> &a,
+                                                      typename uint_of<T>::type *dshift) {
+  using UT = typename uint_of<T>::type;
+  if (threadIdx.x < 8)
+    dshift[threadIdx.x] = threadIdx.x < a.dict_size ? UT(UT(a.dict[threadIdx.x]) << a.right_bw) : UT(0);
+}
+
 template <typename T, bool NT>
-__global__ __launch_bounds__(BLOCK) void k_alprd_decompress(AlprdArgs<T> a, T *__restrict out) {
+__device__ __forceinline__ void alprd_decompress_unit(const AlprdArgs<T> &a, T *__restrict out,
+                                                      uint64_t wave,
+                                                      const typename uint_of<T>::type *dshift) {
   using UT = typename uint_of<T>::type;
   using L = lanes<UT>;
   constexpr int Q = 64 / L::N;
-  __shared__ UT dshift[8];
-  if (threadIdx.x < 8)
-    dshift[threadIdx.x] = threadIdx.x < a.dict_size ? UT(UT(a.dict[threadIdx.x]) << a.right_bw) : UT(0);
-  __syncthreads();
-  const uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE;
   const UnitMap<UT, 1> u = map_unit<UT, 1>(wave);
   if (u.vec >= a.n_vecs) return;
   T *vout = out + u.vec * VALUES_PER_VECTOR;
@@ -96,6 +99,14 @@ __global__ __launch_bounds__(BLOCK) void k_alprd_decompress(AlprdArgs<T> a, T *_
     const UT v = UT(UT(a.exceptions[eo + k]) << a.right_bw) | right;
     store<T, NT>(vout + pos, __builtin_bit_cast(T, v));
   }
+}
+
+template <typename T, bool NT>
+__global__ __launch_bounds__(BLOCK) void k_alprd_decompress(AlprdArgs<T> a, T *__restrict out) {
+  __shared__ typename uint_of<T>::type dshift[8];
+  alprd_fill_dictionary<T>(a, dshift);
+  __syncthreads();
+  alprd_decompress_unit<T, NT>(a, out, (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE, dshift);
 }
 
 template <typename T, bool NT>

@@ -85,12 +85,19 @@ unsigned reduce_grid(uint64_t n_waves) {
   return unsigned(blocks < 1 ? 1 : (blocks < cap ? blocks : cap));
 }
 
-// Autotuner table: [dtype 32/64][bit width 0..64, 255 = column default].
-static int g_tuning[2][66];
+// Autotuner table: [encoding][bit width 0..64, 65 = default for the encoding].
+constexpr int N_ENCODINGS = 10;
+static int g_tuning[N_ENCODINGS][66];
 static std::once_flag g_tuning_once;
 static void tuning_defaults() {
-  for (int d = 0; d < 2; ++d)
-    for (int b = 0; b < 66; ++b) g_tuning[d][b] = FLSGPU_VARIANT_LANE;
+  // Measured defaults on MI355X (FFOR sweep at 2^28 values): two threads per
+  // lane for 32-bit types, four for 64-bit types, streaming stores.
+  for (int e = 0; e < N_ENCODINGS; ++e) {
+    const bool wide = e == FLSGPU_ENC_BP_U64 || e == FLSGPU_ENC_FFOR_U64 || e == FLSGPU_ENC_ALP_F64 ||
+                      e == FLSGPU_ENC_ALP_EXT_F64 || e == FLSGPU_ENC_ALPRD_F64;
+    for (int b = 0; b < 66; ++b)
+      g_tuning[e][b] = wide ? FLSGPU_VARIANT_QUARTER_LANE_NT : FLSGPU_VARIANT_HALF_LANE_NT;
+  }
 }
 
 }  // namespace flsgpu_hip
@@ -109,27 +116,28 @@ int flsgpu_init(int device) {
   return init_device_locked(device);
 }
 
-int flsgpu_set_tuning(uint32_t dtype_bits, uint32_t bit_width, int variant) {
+int flsgpu_set_tuning(uint32_t encoding, uint32_t bit_width, int variant) {
   std::call_once(g_tuning_once, tuning_defaults);
-  if (dtype_bits != 32 && dtype_bits != 64) return set_error(FLSGPU_EINVAL, "dtype_bits must be 32 or 64");
+  if (encoding >= uint32_t(N_ENCODINGS)) return set_error(FLSGPU_EINVAL, "unknown encoding %u", encoding);
   if (variant <= FLSGPU_VARIANT_AUTO || variant >= FLSGPU_VARIANT_COUNT)
     return set_error(FLSGPU_EINVAL, "variant %d out of range", variant);
-  const int d = dtype_bits == 64;
+  const uint32_t w = (encoding == FLSGPU_ENC_BP_U32 || encoding == FLSGPU_ENC_FFOR_U32 ||
+                      encoding == FLSGPU_ENC_ALP_F32 || encoding == FLSGPU_ENC_ALP_EXT_F32 ||
+                      encoding == FLSGPU_ENC_ALPRD_F32) ? 32 : 64;
   if (bit_width == 255) {
-    g_tuning[d][65] = variant;
+    g_tuning[encoding][65] = variant;
   } else {
-    if (bit_width > dtype_bits) return set_error(FLSGPU_EINVAL, "bit width %u > %u", bit_width, dtype_bits);
-    g_tuning[d][bit_width] = variant;
+    if (bit_width > w) return set_error(FLSGPU_EINVAL, "bit width %u > %u", bit_width, w);
+    g_tuning[encoding][bit_width] = variant;
   }
   return FLSGPU_OK;
 }
 
-int flsgpu_get_tuning(uint32_t dtype_bits, uint32_t bit_width) {
+int flsgpu_get_tuning(uint32_t encoding, uint32_t bit_width) {
   std::call_once(g_tuning_once, tuning_defaults);
-  const int d = dtype_bits == 64;
-  if (bit_width == 255) return g_tuning[d][65];
-  if (bit_width > 64) return FLSGPU_VARIANT_LANE;
-  return g_tuning[d][bit_width];
+  if (encoding >= uint32_t(N_ENCODINGS)) return FLSGPU_VARIANT_LANE;
+  if (bit_width == 255 || bit_width > 64) return g_tuning[encoding][65];
+  return g_tuning[encoding][bit_width];
 }
 
 int flsgpu_upload(void **d_ptr, const void *h_ptr, size_t bytes, size_t pad_bytes,

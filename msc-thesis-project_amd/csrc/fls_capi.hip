@@ -13,7 +13,8 @@ struct VariantShape {
 };
 
 static int resolve_variant(int variant, int dtype_bits, VariantShape *vs) {
-  if (variant == FLSGPU_VARIANT_AUTO) variant = flsgpu_get_tuning(dtype_bits, 255);
+  if (variant == FLSGPU_VARIANT_AUTO)
+    variant = flsgpu_get_tuning(dtype_bits == 32 ? FLSGPU_ENC_FFOR_U32 : FLSGPU_ENC_FFOR_U64, 255);
   switch (variant) {
     case FLSGPU_VARIANT_LANE: *vs = {1, false}; return FLSGPU_OK;
     case FLSGPU_VARIANT_LANE_NT: *vs = {1, true}; return FLSGPU_OK;
@@ -74,6 +75,8 @@ static int launch_filter_sum(const FforArgs<UT> &a, int predicate, uint64_t lo, 
   if (predicate == FLSGPU_PRED_OFFSET_LOWER_HALF)
     return FlsFilterSum<UT, 1, FLSGPU_PRED_OFFSET_LOWER_HALF>::launch(a, UT(lo), UT(hi), d_result,
                                                                      stream);
+  if (predicate == FLSGPU_PRED_ALL)
+    return FlsFilterSum<UT, 1, FLSGPU_PRED_ALL>::launch(a, UT(lo), UT(hi), d_result, stream);
   return set_error(FLSGPU_EINVAL, "unknown predicate %d", predicate);
 }
 

@@ -28,6 +28,11 @@ struct DecompressPolicy {
       store<UT, NT>(out + j * lanes<UT>::N, UT(v + base));
     });
   }
+  __device__ __forceinline__ void run_generic(int bw, int j0, int nj) {
+    unpack_runtime<UT>(in, bw, j0, nj, [&](int j, UT v) {
+      store<UT, NT>(out + j * lanes<UT>::N, UT(v + base));
+    });
+  }
 };
 
 template <typename UT>
@@ -40,8 +45,16 @@ struct AnyPolicy {
   __device__ __forceinline__ void run() {
     unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT v) { found |= UT(v + base) == magic; });
   }
+  __device__ __forceinline__ void run_generic(int bw, int j0, int nj) {
+    unpack_runtime<UT>(in, bw, j0, nj, [&](int, UT v) { found |= UT(v + base) == magic; });
+  }
 };
 
+// Fused filter + sum. With no wrap-around in (offset + base) the sum of the
+// passing values is sum(offsets) + count * base, so the per-value work is an
+// unsigned range test and one narrow add on the FOR offset; the base is
+// applied once per lane (the ALU cost per value, not HBM, otherwise bounds
+// this kernel). Columns whose base + 2^BW - 1 wraps take the per-value path.
 template <typename UT, int PRED>
 struct FilterSumPolicy {
   const UT *in;
@@ -50,23 +63,77 @@ struct FilterSumPolicy {
   uint64_t count, sum;
   template <int BW, int J0, int NJ>
   __device__ __forceinline__ void run() {
+    constexpr int W = lanes<UT>::W;
+    constexpr UT MAXOFF = BW == W ? UT(~UT(0)) : UT((UT(1) << BW) - 1);
+    // per-lane offset sums stay below 2^32 when BW + log2(NJ) <= 32
+    using Acc = std::conditional_t<(W == 32 && BW + 6 <= 32), uint32_t, uint64_t>;
+    const bool no_wrap = base <= UT(UT(~UT(0)) - MAXOFF);
     if constexpr (PRED == FLSGPU_PRED_OFFSET_LOWER_HALF) {
       if constexpr (BW > 0) {  // no row passes at bit width 0
-        constexpr UT HALF = UT(UT(1) << (BW - 1));
-        unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT off) {
-          const bool pass = off < HALF;
-          count += pass;
-          sum += pass ? uint64_t(UT(off + base)) : 0;
-        });
+        if (W == 64 || no_wrap) {  // 64-bit sums are mod 2^64 anyway
+          Acc acc = 0;
+          uint32_t cnt = 0;
+          unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT off) {
+            const bool pass = (off >> (BW - 1)) == 0;
+            cnt += pass;
+            acc += pass ? Acc(off) : Acc(0);
+          });
+          count += cnt;
+          sum += uint64_t(acc) + uint64_t(cnt) * uint64_t(base);
+        } else {
+          unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT off) {
+            const bool pass = (off >> (BW - 1)) == 0;
+            count += pass;
+            sum += pass ? uint64_t(UT(off + base)) : 0;
+          });
+        }
+      }
+    } else if constexpr (PRED == FLSGPU_PRED_ALL) {
+      if (W == 64 || no_wrap) {
+        Acc acc = 0;
+        unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT off) { acc += Acc(off); });
+        count += NJ;
+        sum += uint64_t(acc) + uint64_t(NJ) * uint64_t(base);
+      } else {
+        unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT off) { sum += uint64_t(UT(off + base)); });
+        count += NJ;
       }
     } else {
-      unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT off) {
-        const UT v = UT(off + base);
-        const bool pass = (v >= lo) & (v < hi);
-        count += pass;
-        sum += pass ? uint64_t(v) : 0;
-      });
+      if (no_wrap) {
+        // lo <= base + off < hi  <=>  off - olo < span (unsigned)
+        const UT from = lo > base ? lo : base;
+        const UT olo = UT(from - base);
+        const UT span = hi > from ? UT(hi - from) : UT(0);
+        Acc acc = 0;
+        uint32_t cnt = 0;
+        unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT off) {
+          const bool pass = UT(off - olo) < span;
+          cnt += pass;
+          acc += pass ? Acc(off) : Acc(0);
+        });
+        count += cnt;
+        sum += uint64_t(acc) + uint64_t(cnt) * uint64_t(base);
+      } else {
+        unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT off) {
+          const UT v = UT(off + base);
+          const bool pass = (v >= lo) & (v < hi);
+          count += pass;
+          sum += pass ? uint64_t(v) : 0;
+        });
+      }
     }
+  }
+  __device__ __forceinline__ void run_generic(int bw, int j0, int nj) {
+    const UT half = bw > 0 ? UT(UT(1) << (bw - 1)) : UT(0);
+    unpack_runtime<UT>(in, bw, j0, nj, [&](int, UT off) {
+      const UT v = UT(off + base);
+      bool pass;
+      if constexpr (PRED == FLSGPU_PRED_OFFSET_LOWER_HALF) pass = off < half;
+      else if constexpr (PRED == FLSGPU_PRED_ALL) pass = true;
+      else pass = (v >= lo) & (v < hi);
+      count += pass;
+      sum += pass ? uint64_t(v) : 0;
+    });
   }
 };
 
@@ -80,6 +147,19 @@ struct ComputePolicy {
   template <int BW, int J0, int NJ>
   __device__ __forceinline__ void run() {
     unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT v) {
+      UT x = UT(v + base);
+      for (uint32_t k = 0; k < reps; ++k) {
+        x *= UT(3);
+        x <<= 3;
+        x += UT(3);
+        x >>= 3;
+        x &= runtime_zero;
+      }
+      found |= x == UT(1);
+    });
+  }
+  __device__ __forceinline__ void run_generic(int bw, int j0, int nj) {
+    unpack_runtime<UT>(in, bw, j0, nj, [&](int, UT v) {
       UT x = UT(v + base);
       for (uint32_t k = 0; k < reps; ++k) {
         x *= UT(3);
@@ -106,15 +186,21 @@ struct MatchMaskPolicy {
       mask |= uint64_t(UT(v + base) == value) << (j - J0);
     });
   }
+  __device__ __forceinline__ void run_generic(int bw, int j0, int nj) {
+    unpack_runtime<UT>(in, bw, j0, nj, [&](int j, UT v) {
+      mask |= uint64_t(UT(v + base) == value) << (j - j0);
+    });
+  }
 };
 
 // ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
+// Decodes wave-unit `wave` (VPW vectors, slot part wave % S) of a column.
 template <typename UT, int S, bool NT>
-__global__ __launch_bounds__(BLOCK) void k_ffor_decompress(FforArgs<UT> a, UT *__restrict out) {
+__device__ __forceinline__ void ffor_decompress_unit(const FforArgs<UT> &a, UT *__restrict out,
+                                                     uint64_t wave) {
   using L = lanes<UT>;
-  const uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE;
   const UnitMap<UT, S> u = map_unit<UT, S>(wave);
   if (u.vec >= a.n_vecs) return;
   DecompressPolicy<UT, NT> pol{a.packed + a.vector_offsets[u.vec] + u.lane,
@@ -125,6 +211,11 @@ __global__ __launch_bounds__(BLOCK) void k_ffor_decompress(FforArgs<UT> a, UT *_
     for (int j = u.part * (L::W / S); j < (u.part + 1) * (L::W / S); ++j)
       out[u.vec * VALUES_PER_VECTOR + u.lane + j * L::N] = UT(0);
   }
+}
+
+template <typename UT, int S, bool NT>
+__global__ __launch_bounds__(BLOCK) void k_ffor_decompress(FforArgs<UT> a, UT *__restrict out) {
+  ffor_decompress_unit<UT, S, NT>(a, out, (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE);
 }
 
 template <typename UT, int S>

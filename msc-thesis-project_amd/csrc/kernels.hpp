@@ -116,19 +116,4 @@ __device__ __forceinline__ void grid_reduce2(uint64_t v0, uint64_t v1, Scratch s
   }
 }
 
-// Value of slot j (runtime) of a lane packed at runtime width bw (bw <= W).
-template <typename UT>
-__device__ __forceinline__ UT extract_runtime(const UT *__restrict in_lane, int bw, int j) {
-  constexpr int W = lanes<UT>::W;
-  constexpr int NL = lanes<UT>::N;
-  if (bw == 0) return UT(0);
-  const int p = j * bw;
-  const int r = p / W;
-  const int s = p % W;
-  UT v = UT(in_lane[r * NL] >> s);
-  if (s + bw > W) v |= UT(in_lane[(r + 1) * NL] << (W - s));
-  if (bw < W) v &= UT((UT(1) << bw) - 1);
-  return v;
-}
-
 }  // namespace flsgpu_hip

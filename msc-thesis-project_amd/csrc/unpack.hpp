@@ -121,21 +121,45 @@ __device__ __forceinline__ bool dispatch_uniform(Policy &pol, int bw, int part) 
   return true;
 }
 
-// Waterfall over the distinct bit widths present in the wavefront: each
-// distinct width runs its specialised body once with the matching lanes
-// active. With one width per wave (the common case) this is a single pass.
-// Returns false for lanes whose width exceeds W (nothing was emitted).
+// Value of slot j (runtime) of a lane packed at runtime width bw (bw <= W),
+// loading the one or two words it spans.
+template <typename UT>
+__device__ __forceinline__ UT extract_runtime(const UT *__restrict in_lane, int bw, int j) {
+  constexpr int W = lanes<UT>::W;
+  constexpr int NL = lanes<UT>::N;
+  if (bw == 0) return UT(0);
+  const int p = j * bw;
+  const int r = p / W;
+  const int s = p % W;
+  UT v = UT(in_lane[r * NL] >> s);
+  if (s + bw > W) v |= UT(in_lane[(r + 1) * NL] << (W - s));
+  if (bw < W) v &= UT((UT(1) << bw) - 1);
+  return v;
+}
+
+// Runtime-width unpacking of slots [j0, j0 + nj): the path for wavefronts
+// whose vectors have different bit widths.
+template <typename UT, class Emit>
+__device__ __forceinline__ void unpack_runtime(const UT *__restrict in, int bw, int j0, int nj,
+                                               Emit &&emit) {
+  for (int j = j0; j < j0 + nj; ++j) emit(j, extract_runtime<UT>(in, bw, j));
+}
+
+// Dispatch on the lane's bit width. When every vector of the wavefront has
+// the same width (always for single-width columns) the specialised body runs
+// from a scalar branch; otherwise the policy's runtime-width path runs. The
+// two are separate on purpose: a divergent loop over specialised bodies would
+// make the register allocator size every body for the loop (measured: 146
+// instead of 54 VGPRs for the 64-bit quarter-lane decoder). Returns false for
+// lanes whose width exceeds W (nothing was emitted).
 template <typename UT, int S, class Policy>
 __device__ __forceinline__ bool dispatch_bw(Policy &pol, int bw, int part) {
-  bool ok = true;
-  for (;;) {
-    const int b = __builtin_amdgcn_readfirstlane(bw);
-    if (b == bw) {
-      ok = dispatch_uniform<UT, S>(pol, b, part);
-      break;
-    }
-  }
-  return ok;
+  constexpr int W = lanes<UT>::W;
+  const int b0 = __builtin_amdgcn_readfirstlane(bw);
+  if (__builtin_amdgcn_ballot_w64(bw != b0) == 0) return dispatch_uniform<UT, S>(pol, b0, part);
+  if (bw > W) return false;
+  pol.run_generic(bw, part * (W / S), W / S);
+  return true;
 }
 
 }  // namespace flsgpu_hip

@@ -43,6 +43,7 @@ PATCHERS = {
 
 PRED_RANGE = 0
 PRED_OFFSET_LOWER_HALF = 1
+PRED_ALL = 2
 
 
 class BPColumn(C.Structure):
@@ -142,11 +143,15 @@ _SIGS = {
 }
 
 
-def header_symbols(path: str = HEADER_PATH) -> list[str]:
-    """Every function name declared in include/flsgpu.h."""
-    text = open(path).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(flsgpu_[a-z0-9_]+)\s*\(", text)))
+def header_symbols(include_dir: str = os.path.dirname(HEADER_PATH)) -> list[str]:
+    """Every function name declared in the C-ABI headers (include/*.h)."""
+    names = set()
+    for f in sorted(os.listdir(include_dir)):
+        if f.endswith(".h"):
+            text = open(os.path.join(include_dir, f)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names.update(re.findall(r"\b(flsgpu_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def lib() -> C.CDLL:

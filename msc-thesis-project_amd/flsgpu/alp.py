@@ -321,3 +321,46 @@ def ffor_pack(values: np.ndarray, bit_widths: np.ndarray, bases: np.ndarray | No
     _check_enc(getattr(_enc_lib(), f"flsgpu_ffor_pack_u{w}")(
         _np_ptr(values), nv, _np_ptr(bws), _np_ptr(b), _np_ptr(packed), _np_ptr(offs)))
     return packed[: int(bws.astype(np.int64).sum()) * (1024 // w)], offs
+
+
+@dataclass
+class DeviceAlprdColumn:
+    n_values: int
+    right_bw: int
+    left_bw: int
+    dictionary: np.ndarray
+    right_packed: torch.Tensor
+    left_packed: torch.Tensor
+    exceptions_offsets: torch.Tensor
+    exceptions: torch.Tensor
+    positions: torch.Tensor
+    counts: torch.Tensor
+    width: int
+
+    @property
+    def n_vecs(self):
+        return self.n_values // 1024
+
+    def struct(self) -> A.ALPRDColumn:
+        d = (C.c_uint16 * 8)(*([int(x) for x in self.dictionary] + [0] * (8 - len(self.dictionary))))
+        return A.ALPRDColumn(self.n_values, self.n_vecs, self.right_bw, self.left_bw,
+                             len(self.dictionary), d, _ptr(self.right_packed), _ptr(self.left_packed),
+                             _ptr(self.exceptions_offsets), _ptr(self.exceptions),
+                             _ptr(self.positions), _ptr(self.counts))
+
+    @staticmethod
+    def from_host(col: AlprdHostColumn, device="cuda") -> "DeviceAlprdColumn":
+        return DeviceAlprdColumn(col.n_values, col.right_bw, col.left_bw, col.dictionary,
+                                 to_device(col.right_packed, device), to_device(col.left_packed, device),
+                                 to_device(col.exceptions_offsets, device), to_device(col.exceptions, device),
+                                 to_device(col.positions, device), to_device(col.counts, device), col.width)
+
+
+def alprd_decompress(col: DeviceAlprdColumn, out: torch.Tensor | None = None, variant: int = 0,
+                     stream=None) -> torch.Tensor:
+    if out is None:
+        out = torch.empty(col.n_values, dtype=torch.float32 if col.width == 32 else torch.float64,
+                          device=col.right_packed.device)
+    fn = getattr(A.lib(), f"flsgpu_decompress_alprd_{'f32' if col.width == 32 else 'f64'}")
+    A.check(fn(C.byref(col.struct()), _ptr(out), variant, _stream_ptr(stream)))
+    return out

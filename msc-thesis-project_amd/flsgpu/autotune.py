@@ -1,0 +1,129 @@
+"""Autotuner: measures every decoder variant per (encoding, bit width) on the
+current GPU and keeps the fastest, replacing the reference's offline
+"fastest unpacker per bit width" SQL query over benchmark CSVs
+(sql/select-best-unpacker-ffor.sql:1-7). The table is JSON; load_table()
+pushes it into the library (flsgpu_set_tuning) so FLSGPU_VARIANT_AUTO picks it
+up, and variant_for() serves hosts that know a column's bit width.
+
+Run:  python -m flsgpu.autotune --out <table.json>
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+
+import numpy as np
+import torch
+
+from . import _capi as A
+
+DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning_mi355x.json")
+ENC = {"ffor_u32": 1, "ffor_u64": 3, "alp_f32": 4, "alp_f64": 5, "alp_ext_f32": 6,
+       "alp_ext_f64": 7, "alprd_f32": 8, "alprd_f64": 9}
+_table_cache: dict | None = None
+
+
+def _time(fn, reps=10):
+    fn()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def tune_ffor(values_log2: int = 24, reps: int = 10, device="cuda"):
+    from . import device as D
+    from . import encode as E
+    out = {}
+    n = 1 << values_log2
+    for width in (32, 64):
+        key = f"ffor_u{width}"
+        res = torch.empty(n, dtype=torch.uint32 if width == 32 else torch.uint64, device=device)
+        table = {}
+        for bw in range(width + 1):
+            cols = [D.DeviceFFORColumn(n, *E.synthetic_ffor_column_device(n // 1024, bw, width, seed=s),
+                                       width) for s in range(2)]
+            best = None
+            for name, v in A.VARIANTS.items():
+                k = [0]
+
+                def run():
+                    D.decompress(cols[k[0] % 2], res, variant=v)
+                    k[0] += 1
+                ms = _time(run, reps)
+                if best is None or ms < best[1]:
+                    best = (v, ms)
+            table[str(bw)] = best[0]
+        out[key] = table
+    return out
+
+
+def tune_alp(reps: int = 10, device="cuda"):
+    from . import alp as AL
+    from . import workloads as WL
+    out = {}
+    rng = np.random.default_rng(7)
+    for width in (32, 64):
+        x = WL.alp_decimal_values(width, 1 << 22, rng)
+        col = AL.alp_encode(x)
+        for name, c in ((f"alp_f{width}", col), (f"alp_ext_f{width}", AL.galp_convert(col))):
+            d = AL.DeviceAlpColumn.from_host(c, device)
+            res = torch.empty(len(x), dtype=torch.float32 if width == 32 else torch.float64, device=device)
+            best = None
+            for v in A.VARIANTS.values():
+                ms = _time(lambda: AL.decompress(d, res, variant=v), reps)
+                if best is None or ms < best[1]:
+                    best = (v, ms)
+            out[name] = {"255": best[0]}
+    return out
+
+
+def load_table(path: str = DEFAULT_TABLE) -> dict:
+    """Reads a tuning table and loads it into the library."""
+    global _table_cache
+    with open(path) as f:
+        table = json.load(f)
+    L = A.lib()
+    for enc, entries in table.items():
+        if enc not in ENC:
+            continue
+        for bw, v in entries.items():
+            A.check(L.flsgpu_set_tuning(ENC[enc], int(bw), int(v)))
+    _table_cache = table
+    return table
+
+
+def variant_for(encoding: str, bit_width: int) -> int:
+    """Tuned variant for a column whose bit width the host knows."""
+    t = _table_cache or {}
+    e = t.get(encoding, {})
+    if str(bit_width) in e:
+        return int(e[str(bit_width)])
+    return int(A.lib().flsgpu_get_tuning(ENC[encoding], 255))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=DEFAULT_TABLE)
+    ap.add_argument("--values-log2", type=int, default=24)
+    args = ap.parse_args()
+    A.check(A.lib().flsgpu_init(0))
+    table = tune_ffor(args.values_log2)
+    table.update(tune_alp())
+    # the column default is the variant that wins most widths
+    for enc in ("ffor_u32", "ffor_u64"):
+        vals = list(table[enc].values())
+        table[enc]["255"] = max(set(vals), key=vals.count)
+    with open(args.out, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    print(json.dumps(table))
+
+
+if __name__ == "__main__":
+    main()

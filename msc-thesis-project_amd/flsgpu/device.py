@@ -33,21 +33,20 @@ def _stream_ptr(stream) -> int | None:
 
 
 def to_device(a: np.ndarray, device="cuda") -> torch.Tensor:
-    """Upload a numpy array (any length, including 0) as a 1-D device tensor."""
+    """Upload a numpy array as a 1-D device tensor. An empty array becomes a
+    one-element zero tensor, so every column field has a valid device address
+    (e.g. the packed array of a column whose vectors all have bit width 0)."""
     a = np.ascontiguousarray(a)
-    t = torch.from_numpy(a.view(np.uint8).copy()) if a.size else torch.zeros(0, dtype=torch.uint8)
-    t = t.to(device)
-    if a.size:
-        t = t.view(_TORCH_OF[a.dtype])
-    else:
-        t = torch.zeros(0, dtype=_TORCH_OF[a.dtype], device=device)
-    return t
+    if a.size == 0:
+        a = np.zeros(1, dtype=a.dtype)
+    t = torch.from_numpy(a.view(np.uint8).copy()).to(device)
+    return t.view(_TORCH_OF[a.dtype])
 
 
 def _ptr(t: torch.Tensor | None) -> int | None:
     if t is None:
         return None
-    return t.data_ptr() if t.numel() else None
+    return t.data_ptr()
 
 
 @dataclass

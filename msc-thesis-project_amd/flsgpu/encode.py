@@ -92,8 +92,8 @@ def ffor_encode_per_vector(values: np.ndarray):
 # are drawn directly on the GPU.
 # ---------------------------------------------------------------------------
 def random_words_device(n_words: int, width: int, gen: torch.Generator, device="cuda"):
-    if n_words == 0:
-        return torch.zeros(0, dtype=torch.uint32 if width == 32 else torch.uint64, device=device)
+    if n_words == 0:  # keep a valid device address for all-zero-width columns
+        return torch.zeros(1, dtype=torch.uint32 if width == 32 else torch.uint64, device=device)
     n32 = n_words * (width // 32)
     t = torch.randint(-(2 ** 31), 2 ** 31, (n32,), dtype=torch.int32, device=device, generator=gen)
     return t.view(torch.uint32) if width == 32 else t.view(torch.uint64)

@@ -183,7 +183,8 @@ def test_multi_column(width):
     nl = 1024 // width
     cols = []
     for c in range(10):
-        x = c3_like(width, n_vecs, rng)
+        x = (rng.integers(-4000, 4000, n_vecs * 1024) / 4).astype(FT[width])
+        x[rng.choice(n_vecs * 1024, 20, replace=False)] = rng.uniform(-1, 1, 20).astype(FT[width])
         x[2 * 1024 + 3 + nl * np.arange(width)] = FT[width](12.5)
         cols.append(AL.alp_encode(x))
     for layout in ("std", "galp"):

@@ -49,9 +49,14 @@ def test_rejects_null_pointers_and_bad_variant():
 
 def test_tuning_table_roundtrip():
     L = A.lib()
-    assert L.flsgpu_set_tuning(32, 13, A.VARIANT_HALF_LANE) == 0
-    assert L.flsgpu_get_tuning(32, 13) == A.VARIANT_HALF_LANE
-    assert L.flsgpu_set_tuning(32, 33, A.VARIANT_LANE) == A.FLSGPU_EINVAL
-    assert L.flsgpu_set_tuning(16, 3, A.VARIANT_LANE) == A.FLSGPU_EINVAL
-    assert L.flsgpu_set_tuning(64, 3, 0) == A.FLSGPU_EINVAL
-    assert L.flsgpu_set_tuning(32, 13, A.VARIANT_LANE) == 0
+    FFOR_U32, FFOR_U64 = 1, 3
+    assert L.flsgpu_get_tuning(FFOR_U32, 255) == A.VARIANT_HALF_LANE_NT
+    assert L.flsgpu_get_tuning(FFOR_U64, 255) == A.VARIANT_QUARTER_LANE_NT
+    assert L.flsgpu_set_tuning(FFOR_U32, 13, A.VARIANT_HALF_LANE) == 0
+    assert L.flsgpu_get_tuning(FFOR_U32, 13) == A.VARIANT_HALF_LANE
+    assert L.flsgpu_set_tuning(FFOR_U32, 33, A.VARIANT_LANE) == A.FLSGPU_EINVAL
+    assert L.flsgpu_set_tuning(FFOR_U64, 64, A.VARIANT_LANE) == 0
+    assert L.flsgpu_set_tuning(10, 3, A.VARIANT_LANE) == A.FLSGPU_EINVAL
+    assert L.flsgpu_set_tuning(FFOR_U64, 3, 0) == A.FLSGPU_EINVAL
+    assert L.flsgpu_set_tuning(FFOR_U32, 13, A.VARIANT_HALF_LANE_NT) == 0
+    assert L.flsgpu_set_tuning(FFOR_U64, 64, A.VARIANT_QUARTER_LANE_NT) == 0

@@ -123,26 +123,36 @@ def test_query_any(width):
 
 @pytest.mark.parametrize("width", [32, 64])
 def test_filter_sum(width):
+    """Count and sum of the rows passing each predicate, against closed forms
+    on the oracle's decode; bases both small (the algebraic no-wrap path) and
+    full-range (offset + base wraps: the per-value path)."""
     rng = np.random.default_rng(12)
-    mask = (1 << width) - 1
-    for bw in (0, 1, 2, 13, width - 1, width):
-        n_vecs = 37
-        packed, bws, offs, bases = random_column(rng, n_vecs, width, [bw] * n_vecs)
-        col = gpu_column(packed, bws, offs, bases, width)
-        dec = oracle_decode(packed, bws, offs, bases, width)
-        off = (dec.reshape(n_vecs, 1024) - bases[:, None]).reshape(-1)
-        # offset predicate: offset < 2^(bw-1), none pass at bw 0
-        sel = (off < (1 << (bw - 1))) if bw > 0 else np.zeros(len(dec), bool)
-        r = D.query_filter_sum(col, A.PRED_OFFSET_LOWER_HALF).cpu().numpy()
-        exp_sum = int(sum(int(x) for x in dec[sel])) & ((1 << 64) - 1)
-        assert int(r[0]) == int(sel.sum()) and int(r[1]) == exp_sum, f"u{width} bw={bw}"
-        # value range predicate
-        lo, hi = sorted(int(x) for x in rng.choice(dec, 2))
-        sel = (dec >= UT[width](lo)) & (dec < UT[width](hi))
-        r = D.query_filter_sum(col, A.PRED_RANGE, lo, hi).cpu().numpy()
-        exp_sum = int(sum(int(x) for x in dec[sel])) & ((1 << 64) - 1)
-        assert int(r[0]) == int(sel.sum()) and int(r[1]) == exp_sum
-    del mask
+    m64 = (1 << 64) - 1
+    for bw in (0, 1, 2, 13, 27, 28, width - 1, width):
+        for small_bases in (True, False):
+            n_vecs = 37
+            packed, bws, offs, bases = random_column(rng, n_vecs, width, [bw] * n_vecs)
+            if small_bases:
+                bases = rng.integers(0, 1 << 20, n_vecs).astype(UT[width])
+            col = gpu_column(packed, bws, offs, bases, width)
+            dec = oracle_decode(packed, bws, offs, bases, width)
+            off = (dec.reshape(n_vecs, 1024) - bases[:, None]).reshape(-1)
+            tag = f"u{width} bw={bw} small_bases={small_bases}"
+            # offset predicate: offset < 2^(bw-1), none pass at bw 0
+            sel = (off < (1 << (bw - 1))) if bw > 0 else np.zeros(len(dec), bool)
+            r = D.query_filter_sum(col, A.PRED_OFFSET_LOWER_HALF).cpu().numpy()
+            assert int(r[0]) == int(sel.sum()), tag
+            assert int(r[1]) == sum(int(x) for x in dec[sel]) & m64, tag
+            # every row
+            r = D.query_filter_sum(col, A.PRED_ALL).cpu().numpy()
+            assert int(r[0]) == len(dec) and int(r[1]) == sum(int(x) for x in dec) & m64, tag
+            # value range predicate, including empty and full ranges
+            for lo, hi in (sorted(int(x) for x in rng.choice(dec, 2)), (0, 0),
+                           (0, (1 << width) - 1), (int(dec.max()), int(dec.min()))):
+                sel = (dec >= UT[width](lo)) & (dec < UT[width](hi))
+                r = D.query_filter_sum(col, A.PRED_RANGE, lo, hi).cpu().numpy()
+                assert int(r[0]) == int(sel.sum()), (tag, lo, hi)
+                assert int(r[1]) == sum(int(x) for x in dec[sel]) & m64, (tag, lo, hi)
 
 
 @pytest.mark.parametrize("width", [32, 64])
