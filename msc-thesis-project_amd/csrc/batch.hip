@@ -200,6 +200,7 @@ void destroy(flsgpu_batch_plan *p) {
   if (!p) return;
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->graph_stream) (void)hipStreamDestroy(p->graph_stream);
   (void)hipFree(p->d_cols);
   (void)hipFree(p->d_task_col);
   (void)hipFree(p->d_queue);
@@ -319,12 +320,21 @@ int flsgpu_batch_graph_launch(flsgpu_batch_plan *plan, flsgpu_stream_t stream) {
   if (!plan) return set_error(FLSGPU_EINVAL, "null plan");
   if (plan->n_tasks == 0) return FLSGPU_OK;
   if (!plan->exec) {
-    int rc = check_hip(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal),
-                       "hipStreamBeginCapture");
-    if (rc) return rc;
-    const int run_rc = flsgpu_batch_run(plan, stream);
+    // Capture on a private non-blocking stream: the caller's stream may be
+    // the null stream, which cannot be captured. The instantiated graph is
+    // then launched on the caller's stream.
+    int rc;
+    if (!plan->graph_stream &&
+        (rc = check_hip(hipStreamCreateWithFlags(&plan->graph_stream, hipStreamNonBlocking),
+                        "hipStreamCreateWithFlags")))
+      return rc;
+    hipStream_t cs = plan->graph_stream;
+    if ((rc = check_hip(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal),
+                        "hipStreamBeginCapture")))
+      return rc;
+    const int run_rc = flsgpu_batch_run(plan, cs);
     hipGraph_t g = nullptr;
-    rc = check_hip(hipStreamEndCapture(stream, &g), "hipStreamEndCapture");
+    rc = check_hip(hipStreamEndCapture(cs, &g), "hipStreamEndCapture");
     if (run_rc) {
       if (g) (void)hipGraphDestroy(g);
       return run_rc;

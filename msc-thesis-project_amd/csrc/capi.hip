@@ -26,6 +26,9 @@ int set_error(int code, const char *fmt, ...) {
 
 int check_hip(hipError_t e, const char *what) {
   if (e == hipSuccess) return FLSGPU_OK;
+  // Reset the runtime's sticky last error so that a later hipGetLastError()
+  // after a kernel launch does not report this call's failure again.
+  (void)hipGetLastError();
   return set_error(FLSGPU_EHIP, "%s: %s", what, hipGetErrorString(e));
 }
 

@@ -34,8 +34,12 @@ struct int_of {
 
 // ALP tables, one copy per compilation unit (no relocatable device code).
 // src/alp/constants.hpp:49-55 (float), :76-144 (double)
-static __constant__ int32_t c_fact_f32[10] = {1,      10,      100,      1000,      10000,
-                                       100000, 1000000, 10000000, 100000000, 1000000000};
+// Index 10 is not in the reference's float table (src/alp/constants.hpp:55,
+// ten entries) and the encoder never emits it; it holds 10^10 mod 2^32 so
+// that a corrupt factor index of 10 reads defined memory.
+static __constant__ int32_t c_fact_f32[11] = {1,       10,       100,       1000,
+                                              10000,   100000,   1000000,   10000000,
+                                              100000000, 1000000000, 1410065408};
 static __constant__ float c_frac_f32[11] = {1.0,      0.1,       0.01,       0.001,       0.0001,     0.00001,
                                      0.000001, 0.0000001, 0.00000001, 0.000000001, 0.0000000001};
 static __constant__ int64_t c_fact_f64[19] = {1LL,

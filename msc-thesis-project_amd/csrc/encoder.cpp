@@ -14,11 +14,14 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <thread>
 #include <type_traits>
 #include <vector>
+
+#include <sched.h>
 
 #include "flsgpu.h"
 #include "flsgpu_host.h"
@@ -35,6 +38,11 @@ struct Alp<float> {
   using UT = uint32_t;
   using ST = int32_t;
   static constexpr int MAX_EXP = 10;
+  // Factor indices 0..9 only: the reference's float factor table has ten
+  // entries (src/alp/constants.hpp:55), so its search reading index 10 at
+  // exp_ref = factor_idx = 10 (encoder.hpp:153-154) is out of bounds. That
+  // one pair is skipped here.
+  static constexpr int MAX_FAC = 9;
   static constexpr uint64_t EXC_BITS = 32;
   static constexpr uint64_t RD_LIMIT = 22 * 32;
   static constexpr float MAGIC = 12582912.0f;
@@ -56,6 +64,7 @@ struct Alp<double> {
   using UT = uint64_t;
   using ST = int64_t;
   static constexpr int MAX_EXP = 18;
+  static constexpr int MAX_FAC = 18;
   static constexpr uint64_t EXC_BITS = 64;
   static constexpr uint64_t RD_LIMIT = 48 * 32;
   static constexpr double MAGIC = 6755399441055744.0;
@@ -183,7 +192,7 @@ uint64_t search_exhaustive(const T *smp, size_t n, int *out_e, int *out_f) {
   uint64_t best = ss * (A::EXC_BITS + POS_BITS) + ss * A::EXC_BITS;
   int fe = 0, ff = 0;
   for (int e = A::MAX_EXP; e >= 0; --e) {
-    for (int f = e; f >= 0; --f) {
+    for (int f = std::min(e, A::MAX_FAC); f >= 0; --f) {
       unsigned exc, ok;
       const uint64_t bits = estimate<T>(smp, n, e, f, &exc, &ok);
       if (ok < 2) continue;
@@ -318,9 +327,22 @@ void fls_pack(const UT *in, UT *out, unsigned bw) {
   }
 }
 
+// Threads for n_threads <= 0: the CPUs this process may run on, capped by
+// OMP_NUM_THREADS when that is set (the usual way a host limits a job).
+int default_threads() {
+  int n = int(std::max(1u, std::thread::hardware_concurrency()));
+  cpu_set_t set;
+  if (sched_getaffinity(0, sizeof(set), &set) == 0) n = std::max(1, CPU_COUNT(&set));
+  if (const char *e = std::getenv("OMP_NUM_THREADS")) {
+    const int m = std::atoi(e);
+    if (m > 0) n = std::min(n, m);
+  }
+  return n;
+}
+
 template <class F>
 void parallel_for(size_t n, int n_threads, F &&f) {
-  if (n_threads <= 0) n_threads = int(std::max(1u, std::thread::hardware_concurrency()));
+  if (n_threads <= 0) n_threads = default_threads();
   n_threads = int(std::min<size_t>(size_t(n_threads), std::max<size_t>(n, 1)));
   if (n_threads <= 1) {
     f(size_t(0), n);
@@ -591,7 +613,7 @@ int flsgpu_alp_encode_with_params_f32(const float *in, uint64_t n_values, const 
   if (!in || !col || !exps || !facs || n_values % 1024)
     return set_err(FLSGPU_EINVAL, "bad argument");
   for (uint64_t v = 0; v < n_values / 1024; ++v)
-    if (exps[v] > Alp<float>::MAX_EXP || facs[v] > exps[v])
+    if (exps[v] > Alp<float>::MAX_EXP || facs[v] > exps[v] || facs[v] > Alp<float>::MAX_FAC)
       return set_err(FLSGPU_EINVAL, "exponent/factor out of range");
   return encode_with_params<float>(in, n_values / 1024, exps, facs, col, n_threads);
 }

@@ -328,31 +328,39 @@ static inline unsigned count_bits_u64(uint64_t x) { return x == 0 ? 0 : 64 - __b
 static inline unsigned count_bits_u32(uint32_t x) { return x == 0 ? 0 : 32 - __builtin_clz(x); }
 
 /* ---- search ------------------------------------------------------------- */
-#define DEFINE_SEARCH(SUF, T, ST, UT, MAXE, EXCSZ, THRESH, CB)                                    \
+#define DEFINE_SEARCH(SUF, T, ST, UT, MAXE, MAXF, EXCSZ, THRESH, CB)                                    \
+  /* Size estimate of one (exponent, factor) on the samples: the body of the  \
+   * inner loop of encoder.hpp:153-181. UINT64_MAX when fewer than two values \
+   * round-trip (the combination is skipped there). */                         \
+  uint64_t orc_alp_estimate_##SUF(const T *smp, size_t samples_size, unsigned exp_ref,             \
+                                  unsigned fac) {                                                  \
+    unsigned exc = 0, nonexc = 0;                                                                  \
+    ST mx = (ST)((UT)1 << (sizeof(UT) * 8 - 1)), mn = (ST)(((UT)1 << (sizeof(UT) * 8 - 1)) - 1);   \
+    for (size_t i = 0; i < samples_size; ++i) {                                                    \
+      const T a = smp[i];                                                                          \
+      const ST e = orc_alp_encode_value_##SUF(a, fac, exp_ref, 1);                                 \
+      const T d = decode_value_##SUF(e, fac, exp_ref);                                             \
+      if (d == a) {                                                                                \
+        ++nonexc;                                                                                  \
+        if (e > mx) mx = e;                                                                        \
+        if (e < mn) mn = e;                                                                        \
+      } else {                                                                                     \
+        ++exc;                                                                                     \
+      }                                                                                            \
+    }                                                                                              \
+    if (nonexc < 2) return UINT64_MAX;                                                             \
+    const uint64_t bits = CB((UT)((UT)mx - (UT)mn));                                               \
+    return (uint64_t)samples_size * bits + (uint64_t)exc * ((EXCSZ) + EXCEPTION_POSITION_SIZE);    \
+  }                                                                                                \
   static uint64_t search_one_##SUF(const T *smp, uint64_t samples_size, uint64_t worst,            \
                                    uint8_t *out_exp, uint8_t *out_fac) {                           \
     uint8_t found_fac = 0, found_exp = 0;                                                          \
     uint64_t sample_est = worst;                                                                   \
     for (int exp_ref = (MAXE); exp_ref >= 0; --exp_ref) {                                          \
-      for (int fac = exp_ref; fac >= 0; --fac) {                                                   \
-        unsigned exc = 0, nonexc = 0;                                                              \
-        ST mx = (ST)((UT)1 << (sizeof(UT) * 8 - 1)), mn = (ST)(((UT)1 << (sizeof(UT) * 8 - 1)) - 1); \
-        for (uint64_t i = 0; i < samples_size; ++i) {                                              \
-          const T a = smp[i];                                                                      \
-          const ST e = orc_alp_encode_value_##SUF(a, (unsigned)fac, (unsigned)exp_ref, 1);         \
-          const T d = decode_value_##SUF(e, (unsigned)fac, (unsigned)exp_ref);                     \
-          if (d == a) {                                                                            \
-            ++nonexc;                                                                              \
-            if (e > mx) mx = e;                                                                    \
-            if (e < mn) mn = e;                                                                    \
-          } else {                                                                                 \
-            ++exc;                                                                                 \
-          }                                                                                        \
-        }                                                                                          \
-        if (nonexc < 2) continue;                                                                  \
-        const uint64_t bits = CB((UT)((UT)mx - (UT)mn));                                           \
+      for (int fac = exp_ref < (MAXF) ? exp_ref : (MAXF); fac >= 0; --fac) {                       \
         const uint64_t est =                                                                       \
-            samples_size * bits + (uint64_t)exc * ((EXCSZ) + EXCEPTION_POSITION_SIZE);             \
+            orc_alp_estimate_##SUF(smp, samples_size, (unsigned)exp_ref, (unsigned)fac);           \
+        if (est == UINT64_MAX) continue;                                                           \
         if (est < sample_est || (est == sample_est && found_exp < exp_ref) ||                      \
             (est == sample_est && found_exp == exp_ref && found_fac < fac)) {                      \
           sample_est = est;                                                                        \
@@ -449,8 +457,12 @@ static inline unsigned count_bits_u32(uint32_t x) { return x == 0 ? 0 : 32 - __b
     *fac = ff;                                                                                     \
   }
 
-DEFINE_SEARCH(f32, float, int32_t, uint32_t, 10, 32, RD_SIZE_THRESHOLD_F32, count_bits_u32)
-DEFINE_SEARCH(f64, double, int64_t, uint64_t, 18, 64, RD_SIZE_THRESHOLD_F64, count_bits_u64)
+/* Floats search factor indices 0..9: the reference's float factor table has
+ * ten entries (constants.hpp:55), so its read of index 10 at exp_ref =
+ * factor_idx = 10 (encoder.hpp:153-154) is out of bounds; that pair is
+ * skipped. */
+DEFINE_SEARCH(f32, float, int32_t, uint32_t, 10, 9, 32, RD_SIZE_THRESHOLD_F32, count_bits_u32)
+DEFINE_SEARCH(f64, double, int64_t, uint64_t, 18, 18, 64, RD_SIZE_THRESHOLD_F64, count_bits_u64)
 
 /* ---- encode one vector -------------------------------------------------- */
 #define DEFINE_ENCODE_VEC(SUF, T, ST, UT, SIGNMASK, EXPMASK, NEGZERO)                              \

@@ -127,6 +127,10 @@ int orc_alp_top_k_f64(const double *sample, size_t n_samples, uint8_t *exps, uin
  * Returns the estimated size in bits; writes the best (exp, fac). */
 uint64_t orc_alp_search_one_f32(const float *sample, size_t n_samples, uint8_t *exp, uint8_t *fac);
 uint64_t orc_alp_search_one_f64(const double *sample, size_t n_samples, uint8_t *exp, uint8_t *fac);
+/* Size estimate (bits) of one (exponent, factor) on the samples, as the
+ * search computes it; UINT64_MAX when fewer than two values round-trip. */
+uint64_t orc_alp_estimate_f32(const float *sample, size_t n_samples, unsigned exp, unsigned fac);
+uint64_t orc_alp_estimate_f64(const double *sample, size_t n_samples, unsigned exp, unsigned fac);
 /* find_best_exponent_factor_from_combinations (encoder.hpp:236-300). */
 void orc_alp_best_combination_f32(const uint8_t *exps, const uint8_t *facs, int k, const float *vec,
                                   uint8_t *exp, uint8_t *fac);

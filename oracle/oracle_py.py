@@ -59,6 +59,9 @@ def lib():
             f = getattr(_lib, f"orc_alp_search_one_{sfx}")
             f.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]
             f.restype = C.c_uint64
+            f = getattr(_lib, f"orc_alp_estimate_{sfx}")
+            f.argtypes = [C.c_void_p, C.c_size_t, C.c_uint, C.c_uint]
+            f.restype = C.c_uint64
             f = getattr(_lib, f"orc_alp_best_combination_{sfx}")
             f.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
             f.restype = None
@@ -366,6 +369,14 @@ def alp_search_one(sample32: np.ndarray):
     a = np.ascontiguousarray(sample32)
     est = getattr(lib(), f"orc_alp_search_one_{sfx}")(_p(a), len(a), _p(e), _p(f))
     return int(e[0]), int(f[0]), int(est)
+
+
+def alp_estimate(sample32: np.ndarray, exp: int, fac: int):
+    """Estimated bits of (exp, fac) on the samples; None if skipped."""
+    sfx = "f32" if sample32.dtype == np.float32 else "f64"
+    a = np.ascontiguousarray(sample32)
+    v = getattr(lib(), f"orc_alp_estimate_{sfx}")(_p(a), len(a), exp, fac)
+    return None if v == 2 ** 64 - 1 else int(v)
 
 
 def alp_best_combination(combos, vec: np.ndarray, use_ref=False):

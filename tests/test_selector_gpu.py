@@ -1,0 +1,169 @@
+"""GPU checks of the learned ALP selector kernels against the float64 CPU
+restatement (oracle/selector_ref.py): features, forward/backward (gradient
+check), Adam, predict; then a short training run that must beat its own
+initialisation and land close to the exhaustive search's compression ratio,
+with a bit-exact GPU decode of the column encoded with its proposals."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
+import selector_ref as S  # noqa: E402
+from flsgpu import alp as AL  # noqa: E402
+from flsgpu import selector as SEL  # noqa: E402
+
+DEV = "cuda"
+
+
+def _samples(dtype, n, seed):
+    rng = np.random.default_rng(seed)
+    x = SEL.generate(n, rng).astype(dtype)
+    # specials and extremes
+    x[0, :4] = [np.nan, np.inf, -np.inf, -0.0]
+    x[1, :] = 0.0
+    x[2, :] = 3.5
+    x[3, :8] = np.finfo(dtype).max
+    x[4, 5] = np.finfo(dtype).tiny
+    return x
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_features_match_oracle(dtype):
+    x = _samples(dtype, 600, 1)
+    # strided layout: 32 samples at stride 32 inside a 1024-value vector
+    col = np.zeros((x.shape[0], 1024), dtype)
+    col[:, ::32] = x
+    got = SEL.features(torch.from_numpy(col.reshape(-1)).to(DEV), x.shape[0], 1024, 32).cpu().numpy()
+    want = S.features(x)
+    assert np.array_equal(got[:, :19], want[:, :19])
+    exact = [22, 23, 24, 26, 27, 28, 29, 30]       # counts and ratios of counts
+    assert np.array_equal(got[:, exact], want[:, exact])
+    np.testing.assert_allclose(got, want, rtol=2e-6, atol=2e-7)
+    # contiguous layout gives the same features
+    got2 = SEL.features(torch.from_numpy(x.reshape(-1)).to(DEV), x.shape[0], 32, 1).cpu().numpy()
+    assert np.array_equal(got, got2)
+
+
+def test_forward_backward_match_float64_oracle():
+    rng = np.random.default_rng(2)
+    n = 777                                  # not a multiple of the block size
+    x = S.features(SEL.generate(n, rng))
+    lab, _ = SEL.label(SEL.generate(n, rng))
+    p = SEL.init_params(seed=3, device="cpu").numpy()
+    p[S.OB1:S.OW2] = rng.normal(0, 0.1, S.N_H)   # non-zero biases too
+    p[S.OB3:] = rng.normal(0, 0.1, S.N_OUT)
+    pd = torch.from_numpy(p).to(DEV)
+    ws = SEL.Workspace(n, DEV)
+    grad, loss = SEL.forward_backward(pd, torch.from_numpy(x).to(DEV),
+                                      torch.from_numpy(lab).to(DEV), ws)
+    loss_o, g_o = S.backward(p, x, lab)
+    _, _, y_o = S.forward(p, x)
+    np.testing.assert_allclose(ws.y.cpu().numpy(), y_o, rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(loss.cpu().numpy(), loss_o, rtol=1e-4, atol=1e-5)
+    g = grad.cpu().numpy()
+    scale = np.abs(g_o).max()
+    assert np.abs(g - g_o).max() <= 1e-4 * scale, np.abs(g - g_o).max() / scale
+    # every parameter block receives gradient
+    for a, b in ((S.OW1, S.OB1), (S.OW2, S.OB2), (S.OW3, S.OB3), (S.OB3, S.N_PARAM)):
+        assert np.abs(g[a:b]).max() > 0
+
+
+def test_adam_matches_oracle():
+    rng = np.random.default_rng(4)
+    p = rng.normal(size=S.N_PARAM).astype(np.float32)
+    m = np.zeros(S.N_PARAM, np.float32)
+    v = np.zeros(S.N_PARAM, np.float32)
+    pd, md, vd = (torch.from_numpy(a.copy()).to(DEV) for a in (p, m, v))
+    pp, mm, vv = p.astype(np.float64), m.astype(np.float64), v.astype(np.float64)
+    lib = SEL._lib()
+    for step in range(1, 4):
+        g = rng.normal(size=S.N_PARAM).astype(np.float32)
+        gd = torch.from_numpy(g).to(DEV)
+        assert lib.flsgpu_sel_adam(pd.data_ptr(), gd.data_ptr(), md.data_ptr(), vd.data_ptr(),
+                                   1e-2, 0.9, 0.999, 1e-8, step, SEL._s()) == 0
+        pp, mm, vv = S.adam(pp, g.astype(np.float64), mm, vv, 1e-2, 0.9, 0.999, 1e-8, step)
+    np.testing.assert_allclose(pd.cpu().numpy(), pp, rtol=1e-5, atol=1e-6)
+
+
+def test_predict_is_argmax_of_forward():
+    rng = np.random.default_rng(5)
+    n = 1000
+    x = torch.from_numpy(S.features(SEL.generate(n, rng))).to(DEV)
+    p = SEL.init_params(seed=8, device=DEV)
+    ws = SEL.Workspace(n, DEV)
+    lib = SEL._lib()
+    assert lib.flsgpu_sel_forward(p.data_ptr(), x.data_ptr(), n, ws.h1.data_ptr(), ws.h2.data_ptr(),
+                                  ws.y.data_ptr(), SEL._s()) == 0
+    e, f, s = SEL.predict(p, x)
+    ee, ff, ss = S.predict_from_logits(ws.y.cpu().numpy())
+    assert np.array_equal(e.cpu().numpy(), ee)
+    assert np.array_equal(f.cpu().numpy(), ff)
+    assert np.array_equal(s.cpu().numpy(), ss)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_select_all_candidates_is_exhaustive_search(dtype):
+    """With every pair as a candidate the verification is the search itself:
+    exponent, factor and estimate equal the oracle's search on each vector."""
+    import oracle_py as O
+    x = _samples(dtype, 300, 9)
+    xd = torch.from_numpy(x.reshape(-1)).to(DEV)
+    p = SEL.init_params(seed=1, device=DEV)
+    feats = SEL.features(xd, x.shape[0], 32, 1)
+    e, f, s, est = SEL.select(p, feats, xd, x.shape[0], 32, 1, n_candidates=1000)
+    e, f, s, est = (t.cpu().numpy() for t in (e, f, s, est))
+    thr = 48 * 32 if dtype == np.float64 else 22 * 32
+    for r in range(x.shape[0]):
+        oe, of, oest = O.alp_search_one(x[r])
+        assert (e[r], f[r], est[r]) == (oe, of, oest), r
+        assert s[r] == int(oest >= thr)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("k", [1, 4])
+def test_select_top_k_matches_oracle(dtype, k):
+    x = _samples(dtype, 200, 10 + k)
+    n = x.shape[0]
+    xd = torch.from_numpy(x.reshape(-1)).to(DEV)
+    p = SEL.init_params(seed=2, device=DEV)
+    feats = SEL.features(xd, n, 32, 1)
+    ws = SEL.Workspace(n, DEV)
+    lib = SEL._lib()
+    assert lib.flsgpu_sel_forward(p.data_ptr(), feats.data_ptr(), n, ws.h1.data_ptr(),
+                                  ws.h2.data_ptr(), ws.y.data_ptr(), SEL._s()) == 0
+    e, f, s, est = (t.cpu().numpy() for t in SEL.select(p, feats, xd, n, 32, 1, n_candidates=k))
+    oe, of, oest, os_ = S.select(ws.y.cpu().numpy(), x, k)
+    assert np.array_equal(e, oe) and np.array_equal(f, of)
+    assert np.array_equal(est, oest) and np.array_equal(s, os_)
+
+
+def test_select_rejects_bad_arguments():
+    from flsgpu import _capi as A
+    x = torch.zeros(1024, dtype=torch.float64, device=DEV)
+    p = SEL.init_params(seed=1, device=DEV)
+    feats = SEL.features(x, 1, 1024, 32)
+    with pytest.raises(A.FlsgpuError):
+        SEL.select(p, feats, x, 1, 1024, 32, n_candidates=0)
+    with pytest.raises(A.FlsgpuError):
+        SEL.select(p, feats, x, 1, 31, 1, n_candidates=4)
+
+
+def test_training_gain_and_lossless_roundtrip():
+    params, info = SEL.train_default(train_log2=16, epochs=8, seed=7, device=DEV, log=None,
+                                     batch=512)
+    res = SEL.evaluate(params, n_vecs=1024, seed=77, device=DEV)
+    init = SEL.evaluate(SEL.init_params(0, DEV), n_vecs=1024, seed=77, device=DEV)
+    print(res)
+    assert res["roundtrip_bit_exact"] and init["roundtrip_bit_exact"]
+    # training pays: the pure proposal beats the untrained network by far
+    assert res["ratio_learned"] > 1.5 * init["ratio_learned"]
+    assert res["exact_label_rate"] > 2 * init["exact_label_rate"]
+    assert res["ratio_learned"] >= 0.95 * res["ratio_search"]
+    assert res["scheme_accuracy"] >= 0.9
+    # verified top-4 proposals: within 1 % of the exhaustive search
+    assert res["ratio_verified"] >= 0.99 * res["ratio_search"]
