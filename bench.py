@@ -157,7 +157,8 @@ def main():
                                f"2^{args.values_log2} values per width, decode to HBM",
                    "values_per_width": nvals, "variant": args.variant},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(),
                      "kernel": "k_ffor_decompress<uint64_t,...> (average over w=0..64)",
                      "avg_launch_us": round(avg_ms * 1e3, 2),
                      "avg_algorithmic_bytes": int(avg_bytes)},
@@ -181,6 +182,25 @@ def main():
     if args.out:
         with open(args.out, "w") as f:
             f.write(line + "\n")
+
+
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "ffor_decompress_u64_sweep_2p28.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the u64 sweep decoder (average over widths),
+    from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in the profile,
+    each scaled by the same counter's reading for a 2 GiB copy; None if the
+    summary is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def copy_bandwidth(torch, dev, stream):

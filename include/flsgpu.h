@@ -149,6 +149,9 @@ int flsgpu_upload(void **d_ptr, const void *h_ptr, size_t bytes, size_t pad_byte
                   flsgpu_stream_t stream);
 int flsgpu_download(void *h_ptr, const void *d_ptr, size_t bytes, flsgpu_stream_t stream);
 int flsgpu_free(void *d_ptr);
+/* Waits for `stream` (NULL: the whole device, as cudaDeviceSynchronize in
+ * kernels::host::*, src/engine/kernels.cuh:106,128,148). */
+int flsgpu_synchronize(flsgpu_stream_t stream);
 
 /* ---- decompress: kernels::host::decompress_column (src/engine/kernels.cuh:94-112),
  *      bindings::decompress_column (src/generated-bindings/kernel-bindings.cuh:11-16,37-175) */
@@ -218,10 +221,11 @@ int flsgpu_compute_ffor_u64(const flsgpu_ffor_column *col, uint32_t n_repetition
                             uint8_t *d_result, int variant, flsgpu_stream_t stream);
 
 /* ---- multi-column query: multi_column::query_multi_column (arity 1..10,
- *      src/generated-bindings/multi-column-device-kernels.cuh:10-443): the
- *      answer is 1 iff some (vector, lane) has, in every slot, all columns
- *      equal to each other and to `value` ... see DESIGN for the exact
- *      reading; columns must have equal n_vecs. n_cols in 1..10. */
+ *      src/generated-bindings/multi-column-device-kernels.cuh:10-443). The
+ *      answer (*d_found = 1) holds iff some (vector, lane) has every slot of
+ *      that lane equal to `value` in every column, i.e. a whole FastLanes
+ *      lane where all columns match; 0 otherwise. Columns must have equal
+ *      n_vecs; n_cols in 1..10. */
 int flsgpu_query_multi_column_ffor_u32(const flsgpu_ffor_column *cols, uint32_t n_cols,
                                        uint32_t value, uint8_t *d_found, int variant,
                                        flsgpu_stream_t stream);

@@ -51,15 +51,20 @@ static int init_device_locked(int dev) {
   if (rc) return rc;
   g_cus[dev] = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   void *p = nullptr;
-  const size_t bytes = sizeof(uint64_t) * 2 * MAX_REDUCE_BLOCKS + 256;
+  const size_t b_partials = sizeof(uint64_t) * 2 * MAX_REDUCE_BLOCKS;
+  const size_t b_groups = sizeof(uint64_t) * 2 * MAX_REDUCE_GROUPS;
+  const size_t b_counters = sizeof(unsigned) * MAX_REDUCE_GROUPS;
+  const size_t bytes = b_partials + b_groups + b_counters + 256;
   if ((rc = check_hip(hipMalloc(&p, bytes), "hipMalloc(scratch)"))) return rc;
   if ((rc = check_hip(hipMemset(p, 0, bytes), "hipMemset(scratch)"))) {
     (void)hipFree(p);
     return rc;
   }
-  g_scratch[dev].partials = static_cast<uint64_t *>(p);
-  g_scratch[dev].counter =
-      reinterpret_cast<unsigned *>(static_cast<char *>(p) + sizeof(uint64_t) * 2 * MAX_REDUCE_BLOCKS);
+  char *c = static_cast<char *>(p);
+  g_scratch[dev].partials = reinterpret_cast<uint64_t *>(c);
+  g_scratch[dev].group_partials = reinterpret_cast<uint64_t *>(c + b_partials);
+  g_scratch[dev].group_counters = reinterpret_cast<unsigned *>(c + b_partials + b_groups);
+  g_scratch[dev].counter = reinterpret_cast<unsigned *>(c + b_partials + b_groups + b_counters);
   return FLSGPU_OK;
 }
 
@@ -83,9 +88,7 @@ int num_cus() {
 
 unsigned reduce_grid(uint64_t n_waves) {
   const uint64_t blocks = (n_waves + (BLOCK / WAVE) - 1) / (BLOCK / WAVE);
-  const uint64_t want = uint64_t(num_cus()) * 8;
-  const uint64_t cap = want < MAX_REDUCE_BLOCKS ? want : MAX_REDUCE_BLOCKS;
-  return unsigned(blocks < 1 ? 1 : (blocks < cap ? blocks : cap));
+  return unsigned(blocks < 1 ? 1 : (blocks < MAX_REDUCE_BLOCKS ? blocks : MAX_REDUCE_BLOCKS));
 }
 
 // Autotuner table: [encoding][bit width 0..64, 65 = default for the encoding].
@@ -174,6 +177,11 @@ int flsgpu_download(void *h_ptr, const void *d_ptr, size_t bytes, flsgpu_stream_
   int rc = check_hip(hipMemcpyAsync(h_ptr, d_ptr, bytes, hipMemcpyDeviceToHost, stream),
                      "hipMemcpyAsync D2H");
   if (rc) return rc;
+  return check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+int flsgpu_synchronize(flsgpu_stream_t stream) {
+  if (!stream) return check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
   return check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
 }
 

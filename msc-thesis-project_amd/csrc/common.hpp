@@ -119,20 +119,27 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
   return v;
 }
 
-// Per-device scratch owned by the library for cross-block reductions.
+// Per-device scratch owned by the library for cross-block reductions: one
+// partial pair per block, one per group of REDUCE_GROUP blocks, a ticket per
+// group and a top-level ticket. Tickets are zero between launches (the last
+// arrival resets its own).
+constexpr int MAX_REDUCE_BLOCKS = 65536;
+constexpr int REDUCE_GROUP = 64;  // == WAVE: one group is reduced by one wave
+constexpr int MAX_REDUCE_GROUPS = MAX_REDUCE_BLOCKS / REDUCE_GROUP;
 struct Scratch {
-  uint64_t *partials;   // [MAX_REDUCE_BLOCKS * 2]
-  unsigned *counter;    // zero between launches (the last block resets it)
+  uint64_t *partials;        // [MAX_REDUCE_BLOCKS * 2]
+  uint64_t *group_partials;  // [MAX_REDUCE_GROUPS * 2]
+  unsigned *group_counters;  // [MAX_REDUCE_GROUPS]
+  unsigned *counter;         // top-level ticket
 };
-constexpr int MAX_REDUCE_BLOCKS = 2048;
 
 // Status helpers (capi.hip)
 int set_error(int code, const char *fmt, ...);
 int check_hip(hipError_t e, const char *what);
 int get_scratch(Scratch *s);
 int num_cus();
-// Grid for grid-stride reduction kernels: enough blocks to fill the chip,
-// at most MAX_REDUCE_BLOCKS partials.
+// Grid for grid-stride reduction kernels: one wave per unit of work, at
+// most MAX_REDUCE_BLOCKS blocks.
 unsigned reduce_grid(uint64_t n_waves);
 
 }  // namespace flsgpu_hip

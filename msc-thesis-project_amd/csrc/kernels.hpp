@@ -58,14 +58,19 @@ __device__ __forceinline__ uint64_t wave_reduce(uint64_t v) {
   return v;
 }
 
-// Grid-wide reduction of two values: every block reduces in LDS and writes a
-// partial; the last block to finish (atomic ticket) reduces the partials in a
-// fixed order and calls done(total0, total1) from thread 0, then re-arms the
-// ticket for the next launch. Deterministic for any grid size.
+// Grid-wide reduction of two values in two levels, so that no counter sees
+// more than REDUCE_GROUP (or gridDim / REDUCE_GROUP) arrivals: every block
+// reduces in LDS and writes a partial; the last block of each group of
+// REDUCE_GROUP blocks (atomic ticket per group) reduces the group's partials
+// and writes a group partial; the last group (top ticket) reduces those and
+// calls done(total0, total1) from thread 0. Each last arrival re-arms its
+// ticket, so consecutive launches need no reset. Add/Or/And on integers are
+// order-independent, so the result is deterministic.
 template <RedOp OP, class Done>
 __device__ __forceinline__ void grid_reduce2(uint64_t v0, uint64_t v1, Scratch s, Done &&done) {
+  static_assert(REDUCE_GROUP == WAVE, "one wave reduces one group");
   __shared__ uint64_t sh[BLOCK / WAVE][2];
-  __shared__ int am_last;
+  __shared__ int am_last, am_top;
   v0 = wave_reduce<OP>(v0);
   v1 = wave_reduce<OP>(v1);
   const int w = threadIdx.x / WAVE;
@@ -73,7 +78,11 @@ __device__ __forceinline__ void grid_reduce2(uint64_t v0, uint64_t v1, Scratch s
     sh[w][0] = v0;
     sh[w][1] = v1;
   }
+  if (threadIdx.x == 0) am_top = 0;
   __syncthreads();
+  const unsigned g = blockIdx.x / REDUCE_GROUP;
+  const unsigned g0 = g * REDUCE_GROUP;
+  const unsigned gsize = gridDim.x - g0 < unsigned(REDUCE_GROUP) ? gridDim.x - g0 : REDUCE_GROUP;
   if (threadIdx.x == 0) {
     uint64_t a = red_identity<OP>(), b = red_identity<OP>();
     for (int i = 0; i < BLOCK / WAVE; ++i) {
@@ -84,18 +93,44 @@ __device__ __forceinline__ void grid_reduce2(uint64_t v0, uint64_t v1, Scratch s
     __hip_atomic_store(&s.partials[2 * blockIdx.x + 1], b, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
     __threadfence();
-    const unsigned prev = atomicAdd(s.counter, 1u);
-    am_last = (prev == gridDim.x - 1);
+    const unsigned prev = atomicAdd(&s.group_counters[g], 1u);
+    am_last = (prev == gsize - 1);
   }
   __syncthreads();
   if (!am_last) return;
   __threadfence();
+  // Group level: wave 0 holds one partial per lane.
+  if (w == 0) {
+    uint64_t a = red_identity<OP>(), b = red_identity<OP>();
+    if (threadIdx.x < gsize) {
+      const unsigned i = g0 + threadIdx.x;
+      a = __hip_atomic_load(&s.partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      b = __hip_atomic_load(&s.partials[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    a = wave_reduce<OP>(a);
+    b = wave_reduce<OP>(b);
+    if (threadIdx.x == 0) {
+      __hip_atomic_store(&s.group_counters[g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&s.group_partials[2 * g], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&s.group_partials[2 * g + 1], b, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      __threadfence();
+      const unsigned n_groups = (gridDim.x + REDUCE_GROUP - 1) / REDUCE_GROUP;
+      const unsigned prev = atomicAdd(s.counter, 1u);
+      am_top = (prev == n_groups - 1);
+    }
+  }
+  __syncthreads();
+  if (!am_top) return;
+  __threadfence();
+  // Top level: every group partial, in a fixed order per thread.
+  const unsigned n_groups = (gridDim.x + REDUCE_GROUP - 1) / REDUCE_GROUP;
   uint64_t a = red_identity<OP>(), b = red_identity<OP>();
-  for (unsigned i = threadIdx.x; i < gridDim.x; i += BLOCK) {
+  for (unsigned i = threadIdx.x; i < n_groups; i += BLOCK) {
     a = red_combine<OP>(
-        a, __hip_atomic_load(&s.partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    b = red_combine<OP>(
-        b, __hip_atomic_load(&s.partials[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        a, __hip_atomic_load(&s.group_partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    b = red_combine<OP>(b, __hip_atomic_load(&s.group_partials[2 * i + 1], __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT));
   }
   a = wave_reduce<OP>(a);
   b = wave_reduce<OP>(b);
