@@ -239,6 +239,39 @@ int flsgpu_query_multi_column_alp_f64(const flsgpu_alp_column *cols, uint32_t n_
                                       double value, uint8_t *d_found, int patcher, int variant,
                                       flsgpu_stream_t stream);
 
+/* ---- device-side encode: CURRENTLY DISABLED, every call returns
+ *      FLSGPU_EUNSUPPORTED (its first GPU test run faulted the GPU; cause not
+ *      yet identified). Interface as documented below.
+ *
+ *      Device-side encode (no reference counterpart: the reference encodes on
+ *      the host only, src/alp/alp-bindings.cu:83-206, src/fls/ffor.cpp). The
+ *      output bytes equal the host encoder's: flsgpu_alp_encode_with_params_*
+ *      (encode_simdized + analyze_ffor + FastLanes FFOR, src/alp/encoder.hpp:
+ *      104-115, 302-379) and, for integers, FFOR at base = vector minimum and
+ *      width of (max - min). Asynchronous on `stream`; three launches.
+ *
+ *      The descriptor's arrays are caller-allocated device buffers written by
+ *      the call, sized for the worst case: packed n_vecs * 1024 words;
+ *      exceptions and positions n_vecs * 1024; offsets_counts n_vecs * N_LANES
+ *      (G-ALP only); every per-vector array n_vecs. The call sets n_values and
+ *      n_vecs in the descriptor. d_totals (device, 3 x u64) receives the packed
+ *      words, the exception count and, for G-ALP, 1 if some lane holds more
+ *      than 63 exceptions (the 16-bit header cannot encode it; the column is
+ *      then unusable in that layout). d_workspace: flsgpu_encode_workspace_bytes
+ *      (n_vecs) device bytes. exps / facs: per-vector exponent (<= 10 / 18) and
+ *      factor (<= exponent, <= 9 for floats), device arrays. */
+size_t flsgpu_encode_workspace_bytes(uint64_t n_vecs);
+int flsgpu_alp_encode_device_f32(const float *in, uint64_t n_vecs, const uint8_t *exps,
+                                 const uint8_t *facs, int galp, flsgpu_alp_column *col,
+                                 uint64_t *d_totals, void *d_workspace, flsgpu_stream_t stream);
+int flsgpu_alp_encode_device_f64(const double *in, uint64_t n_vecs, const uint8_t *exps,
+                                 const uint8_t *facs, int galp, flsgpu_alp_column *col,
+                                 uint64_t *d_totals, void *d_workspace, flsgpu_stream_t stream);
+int flsgpu_ffor_encode_device_u32(const uint32_t *in, uint64_t n_vecs, flsgpu_ffor_column *col,
+                                  uint64_t *d_totals, void *d_workspace, flsgpu_stream_t stream);
+int flsgpu_ffor_encode_device_u64(const uint64_t *in, uint64_t n_vecs, flsgpu_ffor_column *col,
+                                  uint64_t *d_totals, void *d_workspace, flsgpu_stream_t stream);
+
 /* ---- batched mixed-encoding decode (no reference counterpart: the
  *      reference launches and synchronises once per column,
  *      src/engine/kernels.cuh:94-112). One persistent launch decodes every

@@ -73,11 +73,133 @@ struct AlpMatchPolicy {
   }
 };
 
-// Calls f(position, value) for every exception of vector `vec` that lies in
-// lane `lane` and in slots [j0, j0 + nj).
+// ---------------------------------------------------------------------------
+// Exception lookup.
+//
+// G-ALP layout: the owner thread reads its lane's header (count << 10 |
+// offset) and its exceptions, PF at a time with every load issued before
+// the first is used (the lane's exceptions are contiguous).
+//
+// Standard layout: a vector's exceptions are one list sorted by position, so
+// a thread cannot find its own without scanning. The wave instead indexes
+// its vectors' lists cooperatively in LDS (coalesced loads of the
+// positions): a 1024-bit map of exception positions per vector and a slot
+// mask per lane. The owner then walks its mask; an exception's index in the
+// list is its rank in the position map (positions are sorted).
+// ---------------------------------------------------------------------------
+constexpr int ALP_PF = 8;
+
+template <typename UT>
+__device__ __forceinline__ void lds_or(UT *p, UT v) {
+  if constexpr (sizeof(UT) == 8)
+    atomicOr(reinterpret_cast<unsigned long long *>(p), static_cast<unsigned long long>(v));
+  else
+    atomicOr(reinterpret_cast<unsigned *>(p), static_cast<unsigned>(v));
+}
+
+template <typename T, class F>
+__device__ __forceinline__ void galp_own_exceptions(const AlpArgs<T> &a, uint64_t vec, int lane,
+                                                    int j0, int nj, F &&f) {
+  constexpr int NL = lanes<typename uint_of<T>::type>::N;
+  const uint64_t eo = a.exceptions_offsets[vec];
+  const unsigned oc = a.offsets_counts[vec * NL + lane];
+  const unsigned count = oc >> 10;
+  const uint64_t first = eo + (oc & 0x3FFu);
+  for (unsigned k0 = 0; k0 < count; k0 += ALP_PF) {
+    unsigned pos[ALP_PF];
+    T val[ALP_PF];
+#pragma unroll
+    for (int i = 0; i < ALP_PF; ++i) {
+      if (k0 + i < count) {
+        pos[i] = a.positions[first + k0 + i];
+        val[i] = a.exceptions[first + k0 + i];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < ALP_PF; ++i) {
+      if (k0 + i < count) {
+        const int slot = int(pos[i]) / NL;
+        if (slot >= j0 && slot < j0 + nj) f(pos[i], val[i]);
+      }
+    }
+  }
+}
+
+// Per-wave LDS index of the standard-layout exception lists of the VPW
+// vectors of wave group `group`.
+template <typename T>
+struct StdIndex {
+  using UT = typename uint_of<T>::type;
+  static constexpr int NL = lanes<UT>::N;
+  static constexpr int VPW = lanes<UT>::VPW;
+  uint64_t *bits;  // [VPW][16]: bit p set when position p holds an exception
+  UT *lmask;       // [VPW][NL]: bit j set when slot j of the lane holds one
+
+  // All 64 threads of the wave must call this (no early exit before it).
+  __device__ __forceinline__ void build(const AlpArgs<T> &a, uint64_t group) {
+    const int t = threadIdx.x % WAVE;
+    for (int i = t; i < VPW * 16; i += WAVE) bits[i] = 0;
+    for (int i = t; i < VPW * NL; i += WAVE) lmask[i] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int vv = 0; vv < VPW; ++vv) {
+      const uint64_t vec = group * VPW + vv;
+      if (vec >= a.ffor.n_vecs) break;
+      const unsigned count = a.counts[vec];
+      const uint64_t eo = a.exceptions_offsets[vec];
+      for (unsigned k = t; k < count; k += WAVE) {
+        const unsigned pos = a.positions[eo + k] & 1023u;
+        lds_or<uint64_t>(&bits[vv * 16 + (pos >> 6)], uint64_t(1) << (pos & 63));
+        lds_or<UT>(&lmask[vv * NL + pos % NL], UT(UT(1) << (pos / NL)));
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+
+  // Exceptions of (vector vv of the group, lane) in slots [j0, j0 + nj).
+  template <class F>
+  __device__ __forceinline__ void own(const AlpArgs<T> &a, uint64_t vec, int vv, int lane, int j0,
+                                      int nj, F &&f) const {
+    constexpr int W = lanes<UT>::W;
+    UT m = lmask[vv * NL + lane];
+    if (nj < W) m &= UT(((UT(1) << nj) - 1) << j0);
+    if (!m) return;
+    const uint64_t eo = a.exceptions_offsets[vec];
+    const uint64_t *b = bits + vv * 16;
+    while (m) {
+      unsigned pos[ALP_PF], rank[ALP_PF];
+      int n = 0;
+#pragma unroll
+      for (int i = 0; i < ALP_PF; ++i) {
+        if (m) {
+          const int j = int(__builtin_ctzll(uint64_t(m)));
+          m &= UT(m - 1);
+          const unsigned p = unsigned(lane + NL * j);
+          unsigned r = unsigned(__builtin_popcountll(b[p >> 6] & ((1ull << (p & 63)) - 1)));
+          for (unsigned w = 0; w < (p >> 6); ++w) r += unsigned(__builtin_popcountll(b[w]));
+          pos[i] = p;
+          rank[i] = r;
+          n = i + 1;
+        }
+      }
+      T val[ALP_PF];
+#pragma unroll
+      for (int i = 0; i < ALP_PF; ++i)
+        if (i < n) val[i] = a.exceptions[eo + rank[i]];
+#pragma unroll
+      for (int i = 0; i < ALP_PF; ++i)
+        if (i < n) f(pos[i], val[i]);
+    }
+  }
+};
+
+// Per-thread scan of the exception list: the owner thread reads its lane's
+// exceptions one at a time (G-ALP) or scans the vector's list (standard
+// layout). Used by the batched decoder, which keeps this simpler patch path.
 template <typename T, int LAYOUT, class F>
-__device__ __forceinline__ void for_own_exceptions(const AlpArgs<T> &a, uint64_t vec, int lane,
-                                                   int j0, int nj, F &&f) {
+__device__ __forceinline__ void scan_own_exceptions(const AlpArgs<T> &a, uint64_t vec, int lane,
+                                                    int j0, int nj, F &&f) {
   constexpr int NL = lanes<typename uint_of<T>::type>::N;
   const uint64_t eo = a.exceptions_offsets[vec];
   if constexpr (LAYOUT == ALP_GALP) {
@@ -99,25 +221,64 @@ __device__ __forceinline__ void for_own_exceptions(const AlpArgs<T> &a, uint64_t
   }
 }
 
-template <typename T, int S, bool NT, int LAYOUT, bool PATCH>
-__device__ __forceinline__ void alp_decompress_unit(const AlpArgs<T> &a, T *__restrict out,
-                                                    uint64_t wave) {
+// One wave-unit of the batched decoder: decode, then the per-thread scan.
+template <typename T, bool NT, int LAYOUT>
+__device__ __forceinline__ void alp_decompress_unit_scan(const AlpArgs<T> &a, T *__restrict out,
+                                                         uint64_t wave) {
   using UT = typename uint_of<T>::type;
   using L = lanes<UT>;
-  const UnitMap<UT, S> u = map_unit<UT, S>(wave);
+  const UnitMap<UT, 1> u = map_unit<UT, 1>(wave);
   if (u.vec >= a.ffor.n_vecs) return;
   T *vout = out + u.vec * VALUES_PER_VECTOR;
   AlpDecompressPolicy<T, NT> pol{a.ffor.packed + a.ffor.vector_offsets[u.vec] + u.lane,
                                  vout + u.lane, a.ffor.bases[u.vec],
                                  alp_factor<T>(a.factor_indices[u.vec]),
                                  alp_fraction<T>(a.fraction_indices[u.vec])};
-  const int j0 = u.part * (L::W / S);
-  if (!dispatch_bw<UT, S>(pol, a.ffor.bit_widths[u.vec], u.part)) {
-    for (int j = j0; j < j0 + L::W / S; ++j) vout[u.lane + j * L::N] = T(0);
+  if (!dispatch_bw<UT, 1>(pol, a.ffor.bit_widths[u.vec], 0)) {
+    for (int j = 0; j < L::W; ++j) vout[u.lane + j * L::N] = T(0);
   }
-  if constexpr (PATCH) {
-    for_own_exceptions<T, LAYOUT>(a, u.vec, u.lane, j0, L::W / S,
-                                  [&](unsigned pos, T e) { store<T, NT>(vout + pos, e); });
+  scan_own_exceptions<T, LAYOUT>(a, u.vec, u.lane, 0, L::W,
+                                 [&](unsigned pos, T e) { store<T, NT>(vout + pos, e); });
+}
+
+template <typename T, int S, bool NT, int LAYOUT, bool PATCH>
+__device__ __forceinline__ void alp_decompress_unit(const AlpArgs<T> &a, T *__restrict out,
+                                                    uint64_t wave) {
+  using UT = typename uint_of<T>::type;
+  using L = lanes<UT>;
+  const UnitMap<UT, S> u = map_unit<UT, S>(wave);
+  const int j0 = u.part * (L::W / S);
+  if constexpr (PATCH && LAYOUT == ALP_STD) {
+    __shared__ uint64_t s_bits[BLOCK / WAVE][L::VPW * 16];
+    __shared__ UT s_lmask[BLOCK / WAVE][L::VPW * L::N];
+    const int w = threadIdx.x / WAVE;
+    StdIndex<T> idx{s_bits[w], s_lmask[w]};
+    idx.build(a, wave / S);
+    if (u.vec >= a.ffor.n_vecs) return;
+    T *vout = out + u.vec * VALUES_PER_VECTOR;
+    AlpDecompressPolicy<T, NT> pol{a.ffor.packed + a.ffor.vector_offsets[u.vec] + u.lane,
+                                   vout + u.lane, a.ffor.bases[u.vec],
+                                   alp_factor<T>(a.factor_indices[u.vec]),
+                                   alp_fraction<T>(a.fraction_indices[u.vec])};
+    if (!dispatch_bw<UT, S>(pol, a.ffor.bit_widths[u.vec], u.part)) {
+      for (int j = j0; j < j0 + L::W / S; ++j) vout[u.lane + j * L::N] = T(0);
+    }
+    idx.own(a, u.vec, int((threadIdx.x % WAVE) / L::N), u.lane, j0, L::W / S,
+            [&](unsigned pos, T e) { store<T, NT>(vout + pos, e); });
+  } else {
+    if (u.vec >= a.ffor.n_vecs) return;
+    T *vout = out + u.vec * VALUES_PER_VECTOR;
+    AlpDecompressPolicy<T, NT> pol{a.ffor.packed + a.ffor.vector_offsets[u.vec] + u.lane,
+                                   vout + u.lane, a.ffor.bases[u.vec],
+                                   alp_factor<T>(a.factor_indices[u.vec]),
+                                   alp_fraction<T>(a.fraction_indices[u.vec])};
+    if (!dispatch_bw<UT, S>(pol, a.ffor.bit_widths[u.vec], u.part)) {
+      for (int j = j0; j < j0 + L::W / S; ++j) vout[u.lane + j * L::N] = T(0);
+    }
+    if constexpr (PATCH) {
+      galp_own_exceptions<T>(a, u.vec, u.lane, j0, L::W / S,
+                             [&](unsigned pos, T e) { store<T, NT>(vout + pos, e); });
+    }
   }
 }
 
@@ -128,10 +289,11 @@ __global__ __launch_bounds__(BLOCK) void k_alp_decompress(AlpArgs<T> a, T *__res
 }
 
 // Slot-match mask of one (vector, lane) after patching; exceptions replace
-// the decoded value at their slot.
+// the decoded value at their slot. `idx` is the wave's standard-layout
+// index (built for this vector's group) when LAYOUT == ALP_STD.
 template <typename T, int LAYOUT, bool PATCH>
 __device__ __forceinline__ uint64_t alp_lane_match(const AlpArgs<T> &a, uint64_t vec, int lane,
-                                                   T target) {
+                                                   T target, const StdIndex<T> *idx) {
   using UT = typename uint_of<T>::type;
   using L = lanes<UT>;
   AlpMatchPolicy<T> pol{a.ffor.packed + a.ffor.vector_offsets[vec] + lane, a.ffor.bases[vec],
@@ -143,12 +305,29 @@ __device__ __forceinline__ uint64_t alp_lane_match(const AlpArgs<T> &a, uint64_t
   }
   uint64_t m = pol.mask;
   if constexpr (PATCH) {
-    for_own_exceptions<T, LAYOUT>(a, vec, lane, 0, L::W, [&](unsigned pos, T e) {
+    auto patch = [&](unsigned pos, T e) {
       const uint64_t bit = uint64_t(1) << (pos / L::N);
       m = (e == target) ? (m | bit) : (m & ~bit);
-    });
+    };
+    if constexpr (LAYOUT == ALP_GALP) galp_own_exceptions<T>(a, vec, lane, 0, L::W, patch);
+    else idx->own(a, vec, int((threadIdx.x % WAVE) / L::N), lane, 0, L::W, patch);
   }
   return m;
+}
+
+// LDS for the standard-layout index of the calling wave (unused otherwise).
+template <typename T, int LAYOUT, bool PATCH>
+__device__ __forceinline__ StdIndex<T> wave_std_index() {
+  using UT = typename uint_of<T>::type;
+  using L = lanes<UT>;
+  if constexpr (PATCH && LAYOUT == ALP_STD) {
+    __shared__ uint64_t s_bits[BLOCK / WAVE][L::VPW * 16];
+    __shared__ UT s_lmask[BLOCK / WAVE][L::VPW * L::N];
+    const int w = threadIdx.x / WAVE;
+    return StdIndex<T>{s_bits[w], s_lmask[w]};
+  } else {
+    return StdIndex<T>{nullptr, nullptr};
+  }
 }
 
 template <typename T, int LAYOUT, bool PATCH>
@@ -157,10 +336,13 @@ __global__ __launch_bounds__(BLOCK) void k_alp_any(AlpArgs<T> a, T magic, uint8_
   using UT = typename uint_of<T>::type;
   bool found = false;
   const uint64_t stride = uint64_t(gridDim.x) * (BLOCK / WAVE);
+  StdIndex<T> idx = wave_std_index<T, LAYOUT, PATCH>();
   for (uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE; wave < n_waves;
        wave += stride) {
     const UnitMap<UT, 1> u = map_unit<UT, 1>(wave);
-    if (u.vec < a.ffor.n_vecs) found |= alp_lane_match<T, LAYOUT, PATCH>(a, u.vec, u.lane, magic) != 0;
+    if constexpr (PATCH && LAYOUT == ALP_STD) idx.build(a, wave);
+    if (u.vec < a.ffor.n_vecs)
+      found |= alp_lane_match<T, LAYOUT, PATCH>(a, u.vec, u.lane, magic, &idx) != 0;
   }
   grid_reduce2<RedOp::Or>(found, 0, s, [&](uint64_t f, uint64_t) { *d_found = f ? 1 : 0; });
 }
@@ -173,15 +355,18 @@ __global__ __launch_bounds__(BLOCK) void k_alp_multi(MultiAlpArgs<T> a, T value,
   constexpr uint64_t FULL = L::W == 64 ? ~uint64_t(0) : ((uint64_t(1) << L::W) - 1);
   bool found = false;
   const uint64_t stride = uint64_t(gridDim.x) * (BLOCK / WAVE);
+  StdIndex<T> idx = wave_std_index<T, LAYOUT, PATCH>();
   for (uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE; wave < n_waves;
        wave += stride) {
     const UnitMap<UT, 1> u = map_unit<UT, 1>(wave);
-    if (u.vec < a.col[0].ffor.n_vecs) {
-      uint64_t all = FULL;
-      for (uint32_t c = 0; c < a.n_cols; ++c)
-        all &= alp_lane_match<T, LAYOUT, PATCH>(a.col[c], u.vec, u.lane, value);
-      found |= all == FULL;
+    uint64_t all = FULL;
+    for (uint32_t c = 0; c < a.n_cols; ++c) {
+      // every thread of the wave takes part in building the index
+      if constexpr (PATCH && LAYOUT == ALP_STD) idx.build(a.col[c], wave);
+      if (u.vec < a.col[0].ffor.n_vecs)
+        all &= alp_lane_match<T, LAYOUT, PATCH>(a.col[c], u.vec, u.lane, value, &idx);
     }
+    if (u.vec < a.col[0].ffor.n_vecs) found |= all == FULL;
   }
   grid_reduce2<RedOp::Or>(found, 0, s, [&](uint64_t f, uint64_t) { *d_found = f ? 1 : 0; });
 }

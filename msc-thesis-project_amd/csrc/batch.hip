@@ -55,7 +55,7 @@ template <typename T, int LAYOUT, bool NT>
 __device__ __noinline__ void batch_alp(const BatchColumn *c, uint64_t w0, int wib) {
   const AlpArgs<T> &a = *reinterpret_cast<const AlpArgs<T> *>(&c->u);
   for (int k = 0; k < TASK_WAVES / 4; ++k)
-    alp_decompress_unit<T, 1, NT, LAYOUT, true>(a, static_cast<T *>(c->out), w0 + wib + 4 * k);
+    alp_decompress_unit_scan<T, NT, LAYOUT>(a, static_cast<T *>(c->out), w0 + wib + 4 * k);
 }
 
 template <typename T, bool NT>

@@ -66,6 +66,19 @@ __device__ __forceinline__ uint64_t wave_reduce(uint64_t v) {
 // calls done(total0, total1) from thread 0. Each last arrival re-arms its
 // ticket, so consecutive launches need no reset. Add/Or/And on integers are
 // order-independent, so the result is deterministic.
+//
+// Cross-block visibility without __threadfence(): on MI355X an agent-scope
+// fence is an L2 writeback plus an L2 invalidate of the XCD (buffer_wbl2 /
+// buffer_inv), which at one fence per block cost more than the decoding
+// itself. Partials are instead written with
+// agent-scope atomic stores (written through to the device coherence point),
+// the writer waits for their completion (vmcnt) before taking its ticket, and
+// the reader, which only reads after winning the ticket, uses agent-scope
+// atomic loads.
+__device__ __forceinline__ void wait_stores_complete() {
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
 template <RedOp OP, class Done>
 __device__ __forceinline__ void grid_reduce2(uint64_t v0, uint64_t v1, Scratch s, Done &&done) {
   static_assert(REDUCE_GROUP == WAVE, "one wave reduces one group");
@@ -92,13 +105,13 @@ __device__ __forceinline__ void grid_reduce2(uint64_t v0, uint64_t v1, Scratch s
     __hip_atomic_store(&s.partials[2 * blockIdx.x], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(&s.partials[2 * blockIdx.x + 1], b, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    const unsigned prev = atomicAdd(&s.group_counters[g], 1u);
+    wait_stores_complete();
+    const unsigned prev = __hip_atomic_fetch_add(&s.group_counters[g], 1u, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
     am_last = (prev == gsize - 1);
   }
   __syncthreads();
   if (!am_last) return;
-  __threadfence();
   // Group level: wave 0 holds one partial per lane.
   if (w == 0) {
     uint64_t a = red_identity<OP>(), b = red_identity<OP>();
@@ -114,15 +127,15 @@ __device__ __forceinline__ void grid_reduce2(uint64_t v0, uint64_t v1, Scratch s
       __hip_atomic_store(&s.group_partials[2 * g], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&s.group_partials[2 * g + 1], b, __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
-      __threadfence();
+      wait_stores_complete();
       const unsigned n_groups = (gridDim.x + REDUCE_GROUP - 1) / REDUCE_GROUP;
-      const unsigned prev = atomicAdd(s.counter, 1u);
+      const unsigned prev =
+          __hip_atomic_fetch_add(s.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       am_top = (prev == n_groups - 1);
     }
   }
   __syncthreads();
   if (!am_top) return;
-  __threadfence();
   // Top level: every group partial, in a fixed order per thread.
   const unsigned n_groups = (gridDim.x + REDUCE_GROUP - 1) / REDUCE_GROUP;
   uint64_t a = red_identity<OP>(), b = red_identity<OP>();

@@ -25,22 +25,7 @@
 
 #include "flsgpu.hpp"
 
-extern "C" {
-void orc_ffor_column_decode_u32(size_t n_vecs, const uint32_t *packed, const uint8_t *bit_widths,
-                                const uint64_t *vector_offsets, const uint32_t *bases, uint32_t *out);
-void orc_ffor_column_decode_u64(size_t n_vecs, const uint64_t *packed, const uint8_t *bit_widths,
-                                const uint64_t *vector_offsets, const uint64_t *bases, uint64_t *out);
-void orc_alp_column_decode_f32(size_t n_vecs, const uint32_t *packed, const uint8_t *bit_widths,
-                               const uint64_t *vector_offsets, const uint32_t *bases,
-                               const uint8_t *factor_indices, const uint8_t *fraction_indices,
-                               const uint64_t *exceptions_offsets, const float *exceptions,
-                               const uint16_t *positions, const uint16_t *counts, float *out);
-void orc_alp_column_decode_f64(size_t n_vecs, const uint64_t *packed, const uint8_t *bit_widths,
-                               const uint64_t *vector_offsets, const uint64_t *bases,
-                               const uint8_t *factor_indices, const uint8_t *fraction_indices,
-                               const uint64_t *exceptions_offsets, const double *exceptions,
-                               const uint16_t *positions, const uint16_t *counts, double *out);
-}
+#include "oracle.h"  // the CPU checker (oracle/)
 
 namespace {
 
@@ -184,11 +169,11 @@ T *cpu_decode(const flsgpu::host::ALPColumn<T> &c) {
   if constexpr (sizeof(T) == 4)
     orc_alp_column_decode_f32(c.get_n_vecs(), c.ffor.bp.packed_array, c.ffor.bp.bit_widths, vo,
                               c.ffor.bases, c.factor_indices, c.fraction_indices, eo, c.exceptions,
-                              c.positions, c.counts, out);
+                              c.positions, c.counts, nullptr, out);
   else
     orc_alp_column_decode_f64(c.get_n_vecs(), c.ffor.bp.packed_array, c.ffor.bp.bit_widths, vo,
                               c.ffor.bases, c.factor_indices, c.fraction_indices, eo, c.exceptions,
-                              c.positions, c.counts, out);
+                              c.positions, c.counts, nullptr, out);
   return out;
 }
 
