@@ -17,9 +17,10 @@
  * word beyond a vector's own bit_width * N_LANES words, so no over-read
  * padding is needed after the packed array or the exception arrays.
  *
- * Reductions (query_any, filter_sum, compute, multi-column) write their final
- * result to a caller-provided device pointer from the kernel itself; they use
- * a small per-device scratch area owned by the library, so concurrent calls
+ * Reductions (query_any, filter_sum, compute, multi-column) are two launches
+ * on `stream`: the work kernel leaves one partial per block in a small
+ * per-device scratch area owned by the library, and a one-block kernel
+ * writes the final result to the caller's device pointer. Concurrent calls
  * of these reduction entry points on different streams of one device race.
  */
 #ifndef FLSGPU_H
@@ -52,7 +53,13 @@ enum flsgpu_variant {
   FLSGPU_VARIANT_HALF_LANE_NT = 4,
   FLSGPU_VARIANT_QUARTER_LANE = 5,/* four threads per lane */
   FLSGPU_VARIANT_QUARTER_LANE_NT = 6,
-  FLSGPU_VARIANT_COUNT = 7
+  /* decode into LDS, then 16-byte non-temporal stores of each vector's slot
+   * range; ALP exceptions are applied in LDS. 64-bit columns use the quarter
+   * split for both. Needs a 16-byte aligned output (else the plain NT
+   * variant of the same split runs). */
+  FLSGPU_VARIANT_HALF_LANE_STAGED = 7,
+  FLSGPU_VARIANT_QUARTER_LANE_STAGED = 8,
+  FLSGPU_VARIANT_COUNT = 9
 };
 
 /* Reference enums::Patcher values (src/engine/enums.cuh:43-53). */

@@ -66,9 +66,21 @@ static int make_alp_args(const flsgpu_alp_column *c, int layout, AlpArgs<T> *a) 
   return FLSGPU_OK;
 }
 
-static int alp_shape(int variant, int encoding, int *S, bool *nt) {
+static int alp_shape(int variant, int encoding, int *S, bool *nt, bool *staged) {
   if (variant == FLSGPU_VARIANT_AUTO) variant = flsgpu_get_tuning(encoding, 255);
+  *staged = false;
+  const bool wide = encoding == FLSGPU_ENC_ALP_F64 || encoding == FLSGPU_ENC_ALP_EXT_F64;
   switch (variant) {
+    case FLSGPU_VARIANT_HALF_LANE_STAGED:
+      *S = wide ? 4 : 2;
+      *nt = true;
+      *staged = true;
+      return FLSGPU_OK;
+    case FLSGPU_VARIANT_QUARTER_LANE_STAGED:
+      *S = 4;
+      *nt = true;
+      *staged = true;
+      return FLSGPU_OK;
     case FLSGPU_VARIANT_LANE: *S = 1; *nt = false; return FLSGPU_OK;
     case FLSGPU_VARIANT_LANE_NT: *S = 1; *nt = true; return FLSGPU_OK;
     case FLSGPU_VARIANT_HALF_LANE: *S = 2; *nt = false; return FLSGPU_OK;
@@ -86,6 +98,13 @@ static int decompress_s(const AlpArgs<T> &a, T *out, int layout, bool patch, hip
   return AlpDecompress<T, S, NT, ALP_STD, true>::launch(a, out, st);
 }
 
+template <typename T, int S>
+static int decompress_staged(const AlpArgs<T> &a, T *out, int layout, bool patch, hipStream_t st) {
+  if (!patch) return AlpDecompressStaged<T, S, ALP_STD, false>::launch(a, out, st);
+  if (layout == ALP_GALP) return AlpDecompressStaged<T, S, ALP_GALP, true>::launch(a, out, st);
+  return AlpDecompressStaged<T, S, ALP_STD, true>::launch(a, out, st);
+}
+
 template <typename T>
 static int alp_decompress(const flsgpu_alp_column *c, T *out, int layout, int patcher,
                           int variant, hipStream_t st) {
@@ -96,10 +115,16 @@ static int alp_decompress(const flsgpu_alp_column *c, T *out, int layout, int pa
   bool patch;
   if ((rc = patch_mode(layout, patcher, &patch))) return rc;
   int S;
-  bool nt;
+  bool nt, staged;
   const int enc = sizeof(T) == 4 ? (layout == ALP_GALP ? FLSGPU_ENC_ALP_EXT_F32 : FLSGPU_ENC_ALP_F32)
                                  : (layout == ALP_GALP ? FLSGPU_ENC_ALP_EXT_F64 : FLSGPU_ENC_ALP_F64);
-  if ((rc = alp_shape(variant, enc, &S, &nt))) return rc;
+  if ((rc = alp_shape(variant, enc, &S, &nt, &staged))) return rc;
+  if (staged && (reinterpret_cast<uintptr_t>(out) % 16) == 0) {
+    if constexpr (sizeof(T) == 4) {
+      if (S == 2) return decompress_staged<T, 2>(a, out, layout, patch, st);
+    }
+    return decompress_staged<T, 4>(a, out, layout, patch, st);
+  }
   if (S == 1) return nt ? decompress_s<T, 1, true>(a, out, layout, patch, st)
                         : decompress_s<T, 1, false>(a, out, layout, patch, st);
   if (S == 2) return nt ? decompress_s<T, 2, true>(a, out, layout, patch, st)

@@ -34,6 +34,12 @@ template <typename T, int S, bool NT, int LAYOUT, bool PATCH>
 struct AlpDecompress {
   static int launch(const AlpArgs<T> &a, T *out, hipStream_t stream);
 };
+// Decode through LDS, exceptions applied in LDS by the whole wave, 16-byte
+// stores (out must be 16-byte aligned). PATCH false skips the exceptions.
+template <typename T, int S, int LAYOUT, bool PATCH>
+struct AlpDecompressStaged {
+  static int launch(const AlpArgs<T> &a, T *out, hipStream_t stream);
+};
 template <typename T, int LAYOUT, bool PATCH>
 struct AlpAny {
   static int launch(const AlpArgs<T> &a, T magic, uint8_t *d_found, hipStream_t stream);

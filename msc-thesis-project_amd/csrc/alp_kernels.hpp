@@ -17,6 +17,7 @@
 
 #include "alp_impl.hpp"
 #include "kernels.hpp"
+#include "stage.hpp"
 
 namespace flsgpu_hip {
 
@@ -282,6 +283,77 @@ __device__ __forceinline__ void alp_decompress_unit(const AlpArgs<T> &a, T *__re
   }
 }
 
+template <typename T>
+struct AlpStagePolicy {
+  using UT = typename uint_of<T>::type;
+  const UT *in;
+  T *lds;
+  UT base;
+  UT fact;
+  T frac;
+  template <int BW, int J0, int NJ>
+  __device__ __forceinline__ void run() {
+    unpack_slots<UT, BW, J0, NJ>(in, [&](int j, UT v) {
+      lds[(j - J0) * lanes<UT>::N] = alp_value<T>(v, base, fact, frac);
+    });
+  }
+  __device__ __forceinline__ void run_generic(int bw, int j0, int nj) {
+    unpack_runtime<UT>(in, bw, j0, nj, [&](int j, UT v) {
+      lds[(j - j0) * lanes<UT>::N] = alp_value<T>(v, base, fact, frac);
+    });
+  }
+};
+
+// Staged ALP decode of one wave-unit: decode into LDS, then the wave walks
+// each vector's exception list with coalesced loads and overwrites the
+// staged values whose positions fall in its slot range (positions are unique
+// in a vector, so the writes never collide), then 16-byte stores.
+template <typename T, int S, int LAYOUT, bool PATCH>
+__global__ __launch_bounds__(BLOCK) void k_alp_decompress_staged(AlpArgs<T> a, T *__restrict out) {
+  using UT = typename uint_of<T>::type;
+  using L = lanes<UT>;
+  using Sh = StageShape<UT, S>;
+  __shared__ __align__(16) T s_stage[BLOCK / WAVE][L::VPW * Sh::CHUNK];
+  const uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE;
+  const UnitMap<UT, S> u = map_unit<UT, S>(wave);
+  T *lds = s_stage[threadIdx.x / WAVE];
+  const int t = int(threadIdx.x % WAVE), vv = t / L::N;
+  const int j0 = u.part * Sh::NJ;
+  if (u.vec < a.ffor.n_vecs) {
+    AlpStagePolicy<T> pol{a.ffor.packed + a.ffor.vector_offsets[u.vec] + u.lane,
+                          lds + vv * Sh::CHUNK + u.lane, a.ffor.bases[u.vec],
+                          alp_factor<T>(a.factor_indices[u.vec]),
+                          alp_fraction<T>(a.fraction_indices[u.vec])};
+    if (!dispatch_bw<UT, S>(pol, a.ffor.bit_widths[u.vec], u.part)) {
+      for (int jj = 0; jj < Sh::NJ; ++jj) lds[vv * Sh::CHUNK + u.lane + jj * L::N] = T(0);
+    }
+  }
+  if constexpr (PATCH) {
+    wave_lds_sync();
+    const uint64_t group = wave / S;
+    const unsigned lo = unsigned(L::N * j0), hi = unsigned(L::N * (j0 + Sh::NJ));
+    for (int v2 = 0; v2 < L::VPW; ++v2) {
+      const uint64_t vec = group * L::VPW + v2;
+      if (vec >= a.ffor.n_vecs) break;
+      const uint64_t eo = a.exceptions_offsets[vec];
+      unsigned count;
+      if constexpr (LAYOUT == ALP_GALP) {
+        // lanes are concatenated in order: the last lane's offset + count
+        const unsigned oc = a.offsets_counts[vec * L::N + (L::N - 1)];
+        count = (oc & 0x3FFu) + (oc >> 10);
+      } else {
+        count = a.counts[vec];
+      }
+      for (unsigned k = unsigned(t); k < count; k += WAVE) {
+        const unsigned pos = a.positions[eo + k] & 1023u;
+        if (pos >= lo && pos < hi) lds[v2 * Sh::CHUNK + (pos - lo)] = a.exceptions[eo + k];
+      }
+    }
+  }
+  wave_lds_sync();
+  store_staged<T, S>(lds, out, wave / S, u.part, a.ffor.n_vecs);
+}
+
 template <typename T, int S, bool NT, int LAYOUT, bool PATCH>
 __global__ __launch_bounds__(BLOCK) void k_alp_decompress(AlpArgs<T> a, T *__restrict out) {
   alp_decompress_unit<T, S, NT, LAYOUT, PATCH>(a, out,
@@ -331,8 +403,8 @@ __device__ __forceinline__ StdIndex<T> wave_std_index() {
 }
 
 template <typename T, int LAYOUT, bool PATCH>
-__global__ __launch_bounds__(BLOCK) void k_alp_any(AlpArgs<T> a, T magic, uint8_t *d_found,
-                                                  uint64_t n_waves, Scratch s) {
+__global__ __launch_bounds__(BLOCK) void k_alp_any(AlpArgs<T> a, T magic, uint64_t n_waves,
+                                                  Scratch s) {
   using UT = typename uint_of<T>::type;
   bool found = false;
   const uint64_t stride = uint64_t(gridDim.x) * (BLOCK / WAVE);
@@ -344,12 +416,12 @@ __global__ __launch_bounds__(BLOCK) void k_alp_any(AlpArgs<T> a, T magic, uint8_
     if (u.vec < a.ffor.n_vecs)
       found |= alp_lane_match<T, LAYOUT, PATCH>(a, u.vec, u.lane, magic, &idx) != 0;
   }
-  grid_reduce2<RedOp::Or>(found, 0, s, [&](uint64_t f, uint64_t) { *d_found = f ? 1 : 0; });
+  block_partial2<RedOp::Or>(found, 0, s.partials);
 }
 
 template <typename T, int LAYOUT, bool PATCH>
-__global__ __launch_bounds__(BLOCK) void k_alp_multi(MultiAlpArgs<T> a, T value, uint8_t *d_found,
-                                                    uint64_t n_waves, Scratch s) {
+__global__ __launch_bounds__(BLOCK) void k_alp_multi(MultiAlpArgs<T> a, T value, uint64_t n_waves,
+                                                    Scratch s) {
   using UT = typename uint_of<T>::type;
   using L = lanes<UT>;
   constexpr uint64_t FULL = L::W == 64 ? ~uint64_t(0) : ((uint64_t(1) << L::W) - 1);
@@ -368,7 +440,7 @@ __global__ __launch_bounds__(BLOCK) void k_alp_multi(MultiAlpArgs<T> a, T value,
     }
     if (u.vec < a.col[0].ffor.n_vecs) found |= all == FULL;
   }
-  grid_reduce2<RedOp::Or>(found, 0, s, [&](uint64_t f, uint64_t) { *d_found = f ? 1 : 0; });
+  block_partial2<RedOp::Or>(found, 0, s.partials);
 }
 
 template <typename T, int S, bool NT, int LAYOUT, bool PATCH>
@@ -383,6 +455,18 @@ int AlpDecompress<T, S, NT, LAYOUT, PATCH>::launch(const AlpArgs<T> &a, T *out,
   return check_hip(hipGetLastError(), "k_alp_decompress launch");
 }
 
+template <typename T, int S, int LAYOUT, bool PATCH>
+int AlpDecompressStaged<T, S, LAYOUT, PATCH>::launch(const AlpArgs<T> &a, T *out,
+                                                     hipStream_t stream) {
+  using UT = typename uint_of<T>::type;
+  if (a.ffor.n_vecs == 0) return FLSGPU_OK;
+  const uint64_t blocks = (n_waves_for<UT, S>(a.ffor.n_vecs) + (BLOCK / WAVE) - 1) / (BLOCK / WAVE);
+  if (blocks > 0x7FFFFFFFull) return set_error(FLSGPU_EINVAL, "column too large");
+  hipLaunchKernelGGL((k_alp_decompress_staged<T, S, LAYOUT, PATCH>), dim3(unsigned(blocks)),
+                     dim3(BLOCK), 0, stream, a, out);
+  return check_hip(hipGetLastError(), "k_alp_decompress_staged launch");
+}
+
 template <typename T, int LAYOUT, bool PATCH>
 int AlpAny<T, LAYOUT, PATCH>::launch(const AlpArgs<T> &a, T magic, uint8_t *d_found,
                                      hipStream_t stream) {
@@ -391,9 +475,11 @@ int AlpAny<T, LAYOUT, PATCH>::launch(const AlpArgs<T> &a, T magic, uint8_t *d_fo
   int rc = get_scratch(&s);
   if (rc) return rc;
   const uint64_t n_waves = n_waves_for<UT, 1>(a.ffor.n_vecs);
-  hipLaunchKernelGGL((k_alp_any<T, LAYOUT, PATCH>), dim3(reduce_grid(n_waves)), dim3(BLOCK), 0,
-                     stream, a, magic, d_found, n_waves, s);
-  return check_hip(hipGetLastError(), "k_alp_any launch");
+  const unsigned grid = reduce_grid(n_waves);
+  hipLaunchKernelGGL((k_alp_any<T, LAYOUT, PATCH>), dim3(grid), dim3(BLOCK), 0, stream, a, magic,
+                     n_waves, s);
+  if ((rc = check_hip(hipGetLastError(), "k_alp_any launch"))) return rc;
+  return reduce_finish_flag(s, grid, d_found, stream);
 }
 
 template <typename T, int LAYOUT, bool PATCH>
@@ -404,9 +490,11 @@ int AlpMulti<T, LAYOUT, PATCH>::launch(const MultiAlpArgs<T> &a, T value, uint8_
   int rc = get_scratch(&s);
   if (rc) return rc;
   const uint64_t n_waves = n_waves_for<UT, 1>(a.col[0].ffor.n_vecs);
-  hipLaunchKernelGGL((k_alp_multi<T, LAYOUT, PATCH>), dim3(reduce_grid(n_waves)), dim3(BLOCK), 0,
-                     stream, a, value, d_found, n_waves, s);
-  return check_hip(hipGetLastError(), "k_alp_multi launch");
+  const unsigned grid = reduce_grid(n_waves);
+  hipLaunchKernelGGL((k_alp_multi<T, LAYOUT, PATCH>), dim3(grid), dim3(BLOCK), 0, stream, a,
+                     value, n_waves, s);
+  if ((rc = check_hip(hipGetLastError(), "k_alp_multi launch"))) return rc;
+  return reduce_finish_flag(s, grid, d_found, stream);
 }
 
 }  // namespace flsgpu_hip

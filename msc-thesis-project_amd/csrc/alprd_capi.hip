@@ -36,10 +36,13 @@ static int alprd_decompress(const flsgpu_alprd_column *c, T *out, int variant, h
     variant = flsgpu_get_tuning(W == 32 ? FLSGPU_ENC_ALPRD_F32 : FLSGPU_ENC_ALPRD_F64, 255);
   if (variant < FLSGPU_VARIANT_LANE || variant >= FLSGPU_VARIANT_COUNT)
     return set_error(FLSGPU_EUNSUPPORTED, "unknown variant %d", variant);
-  // The non-temporal variants map to non-temporal stores; lane splitting is
-  // not provided for ALP-RD (a lane's four uint16 left lanes are loaded once).
+  // The non-temporal and staged variants map to non-temporal stores; lane
+  // splitting and staging are not provided for ALP-RD (a lane's four uint16
+  // left lanes are loaded once).
   const bool nt = variant == FLSGPU_VARIANT_LANE_NT || variant == FLSGPU_VARIANT_HALF_LANE_NT ||
-                  variant == FLSGPU_VARIANT_QUARTER_LANE_NT;
+                  variant == FLSGPU_VARIANT_QUARTER_LANE_NT ||
+                  variant == FLSGPU_VARIANT_HALF_LANE_STAGED ||
+                  variant == FLSGPU_VARIANT_QUARTER_LANE_STAGED;
   return nt ? AlprdDecompress<T, true>::launch(a, out, st) : AlprdDecompress<T, false>::launch(a, out, st);
 }
 

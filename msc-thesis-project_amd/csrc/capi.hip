@@ -51,20 +51,9 @@ static int init_device_locked(int dev) {
   if (rc) return rc;
   g_cus[dev] = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
   void *p = nullptr;
-  const size_t b_partials = sizeof(uint64_t) * 2 * MAX_REDUCE_BLOCKS;
-  const size_t b_groups = sizeof(uint64_t) * 2 * MAX_REDUCE_GROUPS;
-  const size_t b_counters = sizeof(unsigned) * MAX_REDUCE_GROUPS;
-  const size_t bytes = b_partials + b_groups + b_counters + 256;
+  const size_t bytes = sizeof(uint64_t) * 2 * MAX_REDUCE_BLOCKS;
   if ((rc = check_hip(hipMalloc(&p, bytes), "hipMalloc(scratch)"))) return rc;
-  if ((rc = check_hip(hipMemset(p, 0, bytes), "hipMemset(scratch)"))) {
-    (void)hipFree(p);
-    return rc;
-  }
-  char *c = static_cast<char *>(p);
-  g_scratch[dev].partials = reinterpret_cast<uint64_t *>(c);
-  g_scratch[dev].group_partials = reinterpret_cast<uint64_t *>(c + b_partials);
-  g_scratch[dev].group_counters = reinterpret_cast<unsigned *>(c + b_partials + b_groups);
-  g_scratch[dev].counter = reinterpret_cast<unsigned *>(c + b_partials + b_groups + b_counters);
+  g_scratch[dev].partials = static_cast<uint64_t *>(p);
   return FLSGPU_OK;
 }
 
@@ -89,6 +78,60 @@ int num_cus() {
 unsigned reduce_grid(uint64_t n_waves) {
   const uint64_t blocks = (n_waves + (BLOCK / WAVE) - 1) / (BLOCK / WAVE);
   return unsigned(blocks < 1 ? 1 : (blocks < MAX_REDUCE_BLOCKS ? blocks : MAX_REDUCE_BLOCKS));
+}
+
+// One block: combines the partials in a fixed order per thread, then across
+// threads (deterministic for any grid).
+constexpr int FINISH_BLOCK = 1024;
+template <bool ADD>
+__global__ __launch_bounds__(FINISH_BLOCK) void k_reduce_finish(const uint64_t *__restrict partials,
+                                                                unsigned n, void *out) {
+  __shared__ uint64_t sh[FINISH_BLOCK / WAVE][2];
+  uint64_t a = 0, b = 0;
+  for (unsigned i = threadIdx.x; i < n; i += FINISH_BLOCK) {
+    if constexpr (ADD) {
+      a += partials[2 * i];
+      b += partials[2 * i + 1];
+    } else {
+      a |= partials[2 * i];
+    }
+  }
+  for (int off = WAVE / 2; off > 0; off >>= 1) {
+    const uint64_t xa = __shfl_xor(a, off, WAVE), xb = __shfl_xor(b, off, WAVE);
+    a = ADD ? a + xa : (a | xa);
+    b = ADD ? b + xb : b;
+  }
+  const int w = threadIdx.x / WAVE;
+  if ((threadIdx.x % WAVE) == 0) {
+    sh[w][0] = a;
+    sh[w][1] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t ta = 0, tb = 0;
+    for (int i = 0; i < FINISH_BLOCK / WAVE; ++i) {
+      ta = ADD ? ta + sh[i][0] : (ta | sh[i][0]);
+      tb += sh[i][1];
+    }
+    if constexpr (ADD) {
+      static_cast<uint64_t *>(out)[0] = ta;
+      static_cast<uint64_t *>(out)[1] = tb;
+    } else {
+      *static_cast<uint8_t *>(out) = ta ? 1 : 0;
+    }
+  }
+}
+
+int reduce_finish_flag(const Scratch &s, unsigned n_blocks, uint8_t *d_flag, hipStream_t stream) {
+  hipLaunchKernelGGL(k_reduce_finish<false>, dim3(1), dim3(FINISH_BLOCK), 0, stream, s.partials,
+                     n_blocks, static_cast<void *>(d_flag));
+  return check_hip(hipGetLastError(), "k_reduce_finish launch");
+}
+
+int reduce_finish_pair(const Scratch &s, unsigned n_blocks, uint64_t *d_pair, hipStream_t stream) {
+  hipLaunchKernelGGL(k_reduce_finish<true>, dim3(1), dim3(FINISH_BLOCK), 0, stream, s.partials,
+                     n_blocks, static_cast<void *>(d_pair));
+  return check_hip(hipGetLastError(), "k_reduce_finish launch");
 }
 
 // Autotuner table: [encoding][bit width 0..64, 65 = default for the encoding].

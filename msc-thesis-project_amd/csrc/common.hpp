@@ -120,17 +120,10 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
 }
 
 // Per-device scratch owned by the library for cross-block reductions: one
-// partial pair per block, one per group of REDUCE_GROUP blocks, a ticket per
-// group and a top-level ticket. Tickets are zero between launches (the last
-// arrival resets its own).
+// partial pair per block of the work kernel.
 constexpr int MAX_REDUCE_BLOCKS = 65536;
-constexpr int REDUCE_GROUP = 64;  // == WAVE: one group is reduced by one wave
-constexpr int MAX_REDUCE_GROUPS = MAX_REDUCE_BLOCKS / REDUCE_GROUP;
 struct Scratch {
-  uint64_t *partials;        // [MAX_REDUCE_BLOCKS * 2]
-  uint64_t *group_partials;  // [MAX_REDUCE_GROUPS * 2]
-  unsigned *group_counters;  // [MAX_REDUCE_GROUPS]
-  unsigned *counter;         // top-level ticket
+  uint64_t *partials;  // [MAX_REDUCE_BLOCKS * 2]
 };
 
 // Status helpers (capi.hip)
@@ -141,5 +134,9 @@ int num_cus();
 // Grid for grid-stride reduction kernels: one wave per unit of work, at
 // most MAX_REDUCE_BLOCKS blocks.
 unsigned reduce_grid(uint64_t n_waves);
+// Finishing launches (one block) over the partials of n_blocks blocks:
+// *d_flag = (OR of the first values) != 0; d_pair[0..1] = sums of the pairs.
+int reduce_finish_flag(const Scratch &s, unsigned n_blocks, uint8_t *d_flag, hipStream_t stream);
+int reduce_finish_pair(const Scratch &s, unsigned n_blocks, uint64_t *d_pair, hipStream_t stream);
 
 }  // namespace flsgpu_hip

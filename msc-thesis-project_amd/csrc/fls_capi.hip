@@ -10,18 +10,21 @@ namespace flsgpu_hip {
 struct VariantShape {
   int S;
   bool nt;
+  bool staged;
 };
 
 static int resolve_variant(int variant, int dtype_bits, VariantShape *vs) {
   if (variant == FLSGPU_VARIANT_AUTO)
     variant = flsgpu_get_tuning(dtype_bits == 32 ? FLSGPU_ENC_FFOR_U32 : FLSGPU_ENC_FFOR_U64, 255);
   switch (variant) {
-    case FLSGPU_VARIANT_LANE: *vs = {1, false}; return FLSGPU_OK;
-    case FLSGPU_VARIANT_LANE_NT: *vs = {1, true}; return FLSGPU_OK;
-    case FLSGPU_VARIANT_HALF_LANE: *vs = {2, false}; return FLSGPU_OK;
-    case FLSGPU_VARIANT_HALF_LANE_NT: *vs = {2, true}; return FLSGPU_OK;
-    case FLSGPU_VARIANT_QUARTER_LANE: *vs = {4, false}; return FLSGPU_OK;
-    case FLSGPU_VARIANT_QUARTER_LANE_NT: *vs = {4, true}; return FLSGPU_OK;
+    case FLSGPU_VARIANT_LANE: *vs = {1, false, false}; return FLSGPU_OK;
+    case FLSGPU_VARIANT_LANE_NT: *vs = {1, true, false}; return FLSGPU_OK;
+    case FLSGPU_VARIANT_HALF_LANE: *vs = {2, false, false}; return FLSGPU_OK;
+    case FLSGPU_VARIANT_HALF_LANE_NT: *vs = {2, true, false}; return FLSGPU_OK;
+    case FLSGPU_VARIANT_QUARTER_LANE: *vs = {4, false, false}; return FLSGPU_OK;
+    case FLSGPU_VARIANT_QUARTER_LANE_NT: *vs = {4, true, false}; return FLSGPU_OK;
+    case FLSGPU_VARIANT_HALF_LANE_STAGED: *vs = {dtype_bits == 64 ? 4 : 2, true, true}; return FLSGPU_OK;
+    case FLSGPU_VARIANT_QUARTER_LANE_STAGED: *vs = {4, true, true}; return FLSGPU_OK;
     default: return set_error(FLSGPU_EUNSUPPORTED, "unknown variant %d", variant);
   }
 }
@@ -45,6 +48,12 @@ static int launch_decompress(const FforArgs<UT> &a, UT *out, int variant, hipStr
   VariantShape vs;
   int rc = resolve_variant(variant, lanes<UT>::W, &vs);
   if (rc) return rc;
+  if (vs.staged && (reinterpret_cast<uintptr_t>(out) % 16) == 0) {
+    if constexpr (lanes<UT>::W == 32) {
+      if (vs.S == 2) return FlsDecompressStaged<UT, 2>::launch(a, out, stream);
+    }
+    return FlsDecompressStaged<UT, 4>::launch(a, out, stream);
+  }
   if (vs.S == 1) return vs.nt ? FlsDecompress<UT, 1, true>::launch(a, out, stream)
                               : FlsDecompress<UT, 1, false>::launch(a, out, stream);
   if (vs.S == 2) return vs.nt ? FlsDecompress<UT, 2, true>::launch(a, out, stream)

@@ -27,6 +27,11 @@ template <typename UT, int S, bool NT>
 struct FlsDecompress {
   static int launch(const FforArgs<UT> &a, UT *out, hipStream_t stream);
 };
+// Decode through LDS with 16-byte stores (out must be 16-byte aligned).
+template <typename UT, int S>
+struct FlsDecompressStaged {
+  static int launch(const FforArgs<UT> &a, UT *out, hipStream_t stream);
+};
 template <typename UT, int S>
 struct FlsAny {
   static int launch(const FforArgs<UT> &a, UT magic, uint8_t *d_found, hipStream_t stream);

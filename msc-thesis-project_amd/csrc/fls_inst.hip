@@ -14,6 +14,8 @@ template struct flsgpu_hip::FlsFilterSum<FLS_T, FLS_S, FLS_PRED>;
 template struct flsgpu_hip::FlsCompute<FLS_T>;
 #elif FLS_FAMILY == 4
 template struct flsgpu_hip::FlsMulti<FLS_T>;
+#elif FLS_FAMILY == 5
+template struct flsgpu_hip::FlsDecompressStaged<FLS_T, FLS_S>;
 #else
 #error "unknown FLS_FAMILY"
 #endif

@@ -11,6 +11,7 @@
 
 #include "fls_impl.hpp"
 #include "kernels.hpp"
+#include "stage.hpp"
 
 namespace flsgpu_hip {
 
@@ -32,6 +33,22 @@ struct DecompressPolicy {
     unpack_runtime<UT>(in, bw, j0, nj, [&](int j, UT v) {
       store<UT, NT>(out + j * lanes<UT>::N, UT(v + base));
     });
+  }
+};
+
+// Writes value + base to the lane's column of the wave's LDS chunk: slot j
+// at lds[(j - j0) * N_LANES].
+template <typename UT>
+struct StagePolicy {
+  const UT *in;
+  UT *lds;
+  UT base;
+  template <int BW, int J0, int NJ>
+  __device__ __forceinline__ void run() {
+    unpack_slots<UT, BW, J0, NJ>(in, [&](int j, UT v) { lds[(j - J0) * lanes<UT>::N] = UT(v + base); });
+  }
+  __device__ __forceinline__ void run_generic(int bw, int j0, int nj) {
+    unpack_runtime<UT>(in, bw, j0, nj, [&](int j, UT v) { lds[(j - j0) * lanes<UT>::N] = UT(v + base); });
   }
 };
 
@@ -218,9 +235,32 @@ __global__ __launch_bounds__(BLOCK) void k_ffor_decompress(FforArgs<UT> a, UT *_
   ffor_decompress_unit<UT, S, NT>(a, out, (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE);
 }
 
+// Staged variant: one wave-unit per wave, decoded into LDS, stored 16 bytes
+// per lane.
 template <typename UT, int S>
-__global__ __launch_bounds__(BLOCK) void k_ffor_any(FforArgs<UT> a, UT magic, uint8_t *d_found,
-                                                   uint64_t n_waves, Scratch s) {
+__global__ __launch_bounds__(BLOCK) void k_ffor_decompress_staged(FforArgs<UT> a,
+                                                                 UT *__restrict out) {
+  using L = lanes<UT>;
+  using Sh = StageShape<UT, S>;
+  __shared__ __align__(16) UT s_stage[BLOCK / WAVE][L::VPW * Sh::CHUNK];
+  const uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE;
+  const UnitMap<UT, S> u = map_unit<UT, S>(wave);
+  UT *lds = s_stage[threadIdx.x / WAVE];
+  const int vv = int(threadIdx.x % WAVE) / L::N;
+  if (u.vec < a.n_vecs) {
+    StagePolicy<UT> pol{a.packed + a.vector_offsets[u.vec] + u.lane, lds + vv * Sh::CHUNK + u.lane,
+                        a.bases ? a.bases[u.vec] : UT(0)};
+    if (!dispatch_bw<UT, S>(pol, a.bit_widths[u.vec], u.part)) {
+      for (int jj = 0; jj < Sh::NJ; ++jj) lds[vv * Sh::CHUNK + u.lane + jj * L::N] = UT(0);
+    }
+  }
+  wave_lds_sync();
+  store_staged<UT, S>(lds, out, wave / S, u.part, a.n_vecs);
+}
+
+template <typename UT, int S>
+__global__ __launch_bounds__(BLOCK) void k_ffor_any(FforArgs<UT> a, UT magic, uint64_t n_waves,
+                                                   Scratch s) {
   bool found = false;
   const uint64_t stride = uint64_t(gridDim.x) * (BLOCK / WAVE);
   for (uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE; wave < n_waves;
@@ -233,13 +273,12 @@ __global__ __launch_bounds__(BLOCK) void k_ffor_any(FforArgs<UT> a, UT magic, ui
       else found |= magic == UT(0);
     }
   }
-  grid_reduce2<RedOp::Or>(found, 0, s, [&](uint64_t f, uint64_t) { *d_found = f ? 1 : 0; });
+  block_partial2<RedOp::Or>(found, 0, s.partials);
 }
 
 template <typename UT, int S, int PRED>
 __global__ __launch_bounds__(BLOCK) void k_ffor_filter_sum(FforArgs<UT> a, UT lo, UT hi,
-                                                          uint64_t *d_result, uint64_t n_waves,
-                                                          Scratch s) {
+                                                          uint64_t n_waves, Scratch s) {
   uint64_t count = 0, sum = 0;
   const uint64_t stride = uint64_t(gridDim.x) * (BLOCK / WAVE);
   for (uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE; wave < n_waves;
@@ -253,16 +292,13 @@ __global__ __launch_bounds__(BLOCK) void k_ffor_filter_sum(FforArgs<UT> a, UT lo
       sum += pol.sum;
     }
   }
-  grid_reduce2<RedOp::Add>(count, sum, s, [&](uint64_t c, uint64_t v) {
-    d_result[0] = c;
-    d_result[1] = v;
-  });
+  block_partial2<RedOp::Add>(count, sum, s.partials);
 }
 
 template <typename UT>
 __global__ __launch_bounds__(BLOCK) void k_ffor_compute(FforArgs<UT> a, uint32_t reps,
-                                                       UT runtime_zero, uint8_t *d_found,
-                                                       uint64_t n_waves, Scratch s) {
+                                                       UT runtime_zero, uint64_t n_waves,
+                                                       Scratch s) {
   bool found = false;
   const uint64_t stride = uint64_t(gridDim.x) * (BLOCK / WAVE);
   for (uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE; wave < n_waves;
@@ -275,15 +311,14 @@ __global__ __launch_bounds__(BLOCK) void k_ffor_compute(FforArgs<UT> a, uint32_t
       found |= pol.found;
     }
   }
-  grid_reduce2<RedOp::Or>(found, 0, s, [&](uint64_t f, uint64_t) { *d_found = f ? 1 : 0; });
+  block_partial2<RedOp::Or>(found, 0, s.partials);
 }
 
 // One thread per (vector, lane): the answer is 1 iff some lane has every slot
 // equal to `value` in every column.
 template <typename UT>
 __global__ __launch_bounds__(BLOCK) void k_ffor_multi(MultiFforArgs<UT> a, UT value,
-                                                     uint8_t *d_found, uint64_t n_waves,
-                                                     Scratch s) {
+                                                     uint64_t n_waves, Scratch s) {
   using L = lanes<UT>;
   constexpr uint64_t FULL = L::W == 64 ? ~uint64_t(0) : ((uint64_t(1) << L::W) - 1);
   bool found = false;
@@ -305,7 +340,7 @@ __global__ __launch_bounds__(BLOCK) void k_ffor_multi(MultiFforArgs<UT> a, UT va
       found |= all == FULL;
     }
   }
-  grid_reduce2<RedOp::Or>(found, 0, s, [&](uint64_t f, uint64_t) { *d_found = f ? 1 : 0; });
+  block_partial2<RedOp::Or>(found, 0, s.partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -322,14 +357,26 @@ int FlsDecompress<UT, S, NT>::launch(const FforArgs<UT> &a, UT *out, hipStream_t
 }
 
 template <typename UT, int S>
+int FlsDecompressStaged<UT, S>::launch(const FforArgs<UT> &a, UT *out, hipStream_t stream) {
+  if (a.n_vecs == 0) return FLSGPU_OK;
+  const uint64_t blocks = (n_waves_for<UT, S>(a.n_vecs) + (BLOCK / WAVE) - 1) / (BLOCK / WAVE);
+  if (blocks > 0x7FFFFFFFull) return set_error(FLSGPU_EINVAL, "column too large");
+  hipLaunchKernelGGL((k_ffor_decompress_staged<UT, S>), dim3(unsigned(blocks)), dim3(BLOCK), 0,
+                     stream, a, out);
+  return check_hip(hipGetLastError(), "k_ffor_decompress_staged launch");
+}
+
+template <typename UT, int S>
 int FlsAny<UT, S>::launch(const FforArgs<UT> &a, UT magic, uint8_t *d_found, hipStream_t stream) {
   Scratch s;
   int rc = get_scratch(&s);
   if (rc) return rc;
   const uint64_t n_waves = n_waves_for<UT, S>(a.n_vecs);
-  hipLaunchKernelGGL((k_ffor_any<UT, S>), dim3(reduce_grid(n_waves)), dim3(BLOCK), 0, stream, a,
-                     magic, d_found, n_waves, s);
-  return check_hip(hipGetLastError(), "k_ffor_any launch");
+  const unsigned grid = reduce_grid(n_waves);
+  hipLaunchKernelGGL((k_ffor_any<UT, S>), dim3(grid), dim3(BLOCK), 0, stream, a, magic, n_waves,
+                     s);
+  if ((rc = check_hip(hipGetLastError(), "k_ffor_any launch"))) return rc;
+  return reduce_finish_flag(s, grid, d_found, stream);
 }
 
 template <typename UT, int S, int PRED>
@@ -339,9 +386,11 @@ int FlsFilterSum<UT, S, PRED>::launch(const FforArgs<UT> &a, UT lo, UT hi, uint6
   int rc = get_scratch(&s);
   if (rc) return rc;
   const uint64_t n_waves = n_waves_for<UT, S>(a.n_vecs);
-  hipLaunchKernelGGL((k_ffor_filter_sum<UT, S, PRED>), dim3(reduce_grid(n_waves)), dim3(BLOCK), 0,
-                     stream, a, lo, hi, d_result, n_waves, s);
-  return check_hip(hipGetLastError(), "k_ffor_filter_sum launch");
+  const unsigned grid = reduce_grid(n_waves);
+  hipLaunchKernelGGL((k_ffor_filter_sum<UT, S, PRED>), dim3(grid), dim3(BLOCK), 0, stream, a, lo,
+                     hi, n_waves, s);
+  if ((rc = check_hip(hipGetLastError(), "k_ffor_filter_sum launch"))) return rc;
+  return reduce_finish_pair(s, grid, d_result, stream);
 }
 
 template <typename UT>
@@ -351,9 +400,11 @@ int FlsCompute<UT>::launch(const FforArgs<UT> &a, uint32_t reps, uint8_t *d_foun
   int rc = get_scratch(&s);
   if (rc) return rc;
   const uint64_t n_waves = n_waves_for<UT, 1>(a.n_vecs);
-  hipLaunchKernelGGL((k_ffor_compute<UT>), dim3(reduce_grid(n_waves)), dim3(BLOCK), 0, stream, a,
-                     reps, UT(0), d_found, n_waves, s);
-  return check_hip(hipGetLastError(), "k_ffor_compute launch");
+  const unsigned grid = reduce_grid(n_waves);
+  hipLaunchKernelGGL((k_ffor_compute<UT>), dim3(grid), dim3(BLOCK), 0, stream, a, reps, UT(0),
+                     n_waves, s);
+  if ((rc = check_hip(hipGetLastError(), "k_ffor_compute launch"))) return rc;
+  return reduce_finish_flag(s, grid, d_found, stream);
 }
 
 template <typename UT>
@@ -363,9 +414,11 @@ int FlsMulti<UT>::launch(const MultiFforArgs<UT> &a, UT value, uint8_t *d_found,
   int rc = get_scratch(&s);
   if (rc) return rc;
   const uint64_t n_waves = n_waves_for<UT, 1>(a.col[0].n_vecs);
-  hipLaunchKernelGGL((k_ffor_multi<UT>), dim3(reduce_grid(n_waves)), dim3(BLOCK), 0, stream, a,
-                     value, d_found, n_waves, s);
-  return check_hip(hipGetLastError(), "k_ffor_multi launch");
+  const unsigned grid = reduce_grid(n_waves);
+  hipLaunchKernelGGL((k_ffor_multi<UT>), dim3(grid), dim3(BLOCK), 0, stream, a, value, n_waves,
+                     s);
+  if ((rc = check_hip(hipGetLastError(), "k_ffor_multi launch"))) return rc;
+  return reduce_finish_flag(s, grid, d_found, stream);
 }
 
 }  // namespace flsgpu_hip

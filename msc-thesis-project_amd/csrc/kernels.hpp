@@ -58,32 +58,19 @@ __device__ __forceinline__ uint64_t wave_reduce(uint64_t v) {
   return v;
 }
 
-// Grid-wide reduction of two values in two levels, so that no counter sees
-// more than REDUCE_GROUP (or gridDim / REDUCE_GROUP) arrivals: every block
-// reduces in LDS and writes a partial; the last block of each group of
-// REDUCE_GROUP blocks (atomic ticket per group) reduces the group's partials
-// and writes a group partial; the last group (top ticket) reduces those and
-// calls done(total0, total1) from thread 0. Each last arrival re-arms its
-// ticket, so consecutive launches need no reset. Add/Or/And on integers are
-// order-independent, so the result is deterministic.
-//
-// Cross-block visibility without __threadfence(): on MI355X an agent-scope
-// fence is an L2 writeback plus an L2 invalidate of the XCD (buffer_wbl2 /
-// buffer_inv), which at one fence per block cost more than the decoding
-// itself. Partials are instead written with
-// agent-scope atomic stores (written through to the device coherence point),
-// the writer waits for their completion (vmcnt) before taking its ticket, and
-// the reader, which only reads after winning the ticket, uses agent-scope
-// atomic loads.
-__device__ __forceinline__ void wait_stores_complete() {
-  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-}
-
-template <RedOp OP, class Done>
-__device__ __forceinline__ void grid_reduce2(uint64_t v0, uint64_t v1, Scratch s, Done &&done) {
-  static_assert(REDUCE_GROUP == WAVE, "one wave reduces one group");
+// Cross-block reductions run as two launches: every block of the work
+// kernel reduces its waves in LDS and stores one partial pair
+// (block_partial2); a one-block finishing kernel (reduce_finish_*, capi.hip)
+// combines the partials and writes the result. The kernel boundary orders
+// the partials before the finish, so no block waits on a fence or a ticket:
+// on gfx950 an agent-scope fence is an L2 write-back plus invalidate of the
+// XCD (buffer_wbl2 / buffer_inv), and per-block tickets keep every block
+// alive for a round trip to the coherence point; at small bit widths either
+// cost more than the decoding.
+template <RedOp OP>
+__device__ __forceinline__ void block_partial2(uint64_t v0, uint64_t v1,
+                                               uint64_t *__restrict partials) {
   __shared__ uint64_t sh[BLOCK / WAVE][2];
-  __shared__ int am_last, am_top;
   v0 = wave_reduce<OP>(v0);
   v1 = wave_reduce<OP>(v1);
   const int w = threadIdx.x / WAVE;
@@ -91,76 +78,15 @@ __device__ __forceinline__ void grid_reduce2(uint64_t v0, uint64_t v1, Scratch s
     sh[w][0] = v0;
     sh[w][1] = v1;
   }
-  if (threadIdx.x == 0) am_top = 0;
   __syncthreads();
-  const unsigned g = blockIdx.x / REDUCE_GROUP;
-  const unsigned g0 = g * REDUCE_GROUP;
-  const unsigned gsize = gridDim.x - g0 < unsigned(REDUCE_GROUP) ? gridDim.x - g0 : REDUCE_GROUP;
   if (threadIdx.x == 0) {
     uint64_t a = red_identity<OP>(), b = red_identity<OP>();
     for (int i = 0; i < BLOCK / WAVE; ++i) {
       a = red_combine<OP>(a, sh[i][0]);
       b = red_combine<OP>(b, sh[i][1]);
     }
-    __hip_atomic_store(&s.partials[2 * blockIdx.x], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&s.partials[2 * blockIdx.x + 1], b, __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-    wait_stores_complete();
-    const unsigned prev = __hip_atomic_fetch_add(&s.group_counters[g], 1u, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-    am_last = (prev == gsize - 1);
-  }
-  __syncthreads();
-  if (!am_last) return;
-  // Group level: wave 0 holds one partial per lane.
-  if (w == 0) {
-    uint64_t a = red_identity<OP>(), b = red_identity<OP>();
-    if (threadIdx.x < gsize) {
-      const unsigned i = g0 + threadIdx.x;
-      a = __hip_atomic_load(&s.partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      b = __hip_atomic_load(&s.partials[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    a = wave_reduce<OP>(a);
-    b = wave_reduce<OP>(b);
-    if (threadIdx.x == 0) {
-      __hip_atomic_store(&s.group_counters[g], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&s.group_partials[2 * g], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&s.group_partials[2 * g + 1], b, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      wait_stores_complete();
-      const unsigned n_groups = (gridDim.x + REDUCE_GROUP - 1) / REDUCE_GROUP;
-      const unsigned prev =
-          __hip_atomic_fetch_add(s.counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      am_top = (prev == n_groups - 1);
-    }
-  }
-  __syncthreads();
-  if (!am_top) return;
-  // Top level: every group partial, in a fixed order per thread.
-  const unsigned n_groups = (gridDim.x + REDUCE_GROUP - 1) / REDUCE_GROUP;
-  uint64_t a = red_identity<OP>(), b = red_identity<OP>();
-  for (unsigned i = threadIdx.x; i < n_groups; i += BLOCK) {
-    a = red_combine<OP>(
-        a, __hip_atomic_load(&s.group_partials[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    b = red_combine<OP>(b, __hip_atomic_load(&s.group_partials[2 * i + 1], __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT));
-  }
-  a = wave_reduce<OP>(a);
-  b = wave_reduce<OP>(b);
-  __syncthreads();
-  if ((threadIdx.x % WAVE) == 0) {
-    sh[w][0] = a;
-    sh[w][1] = b;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint64_t ta = red_identity<OP>(), tb = red_identity<OP>();
-    for (int i = 0; i < BLOCK / WAVE; ++i) {
-      ta = red_combine<OP>(ta, sh[i][0]);
-      tb = red_combine<OP>(tb, sh[i][1]);
-    }
-    done(ta, tb);
-    __hip_atomic_store(s.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    partials[2 * blockIdx.x] = a;
+    partials[2 * blockIdx.x + 1] = b;
   }
 }
 

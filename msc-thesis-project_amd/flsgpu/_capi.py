@@ -26,6 +26,8 @@ VARIANT_HALF_LANE = 3
 VARIANT_HALF_LANE_NT = 4
 VARIANT_QUARTER_LANE = 5
 VARIANT_QUARTER_LANE_NT = 6
+VARIANT_HALF_LANE_STAGED = 7
+VARIANT_QUARTER_LANE_STAGED = 8
 VARIANTS = {
     "lane": VARIANT_LANE,
     "lane-nt": VARIANT_LANE_NT,
@@ -33,6 +35,8 @@ VARIANTS = {
     "half-lane-nt": VARIANT_HALF_LANE_NT,
     "quarter-lane": VARIANT_QUARTER_LANE,
     "quarter-lane-nt": VARIANT_QUARTER_LANE_NT,
+    "half-lane-staged": VARIANT_HALF_LANE_STAGED,
+    "quarter-lane-staged": VARIANT_QUARTER_LANE_STAGED,
 }
 
 PATCHERS = {

@@ -111,10 +111,31 @@ def test_many_exceptions_per_lane(width):
     col.positions = np.concatenate(pos_l)
     col.exceptions = rand_bits(rng, len(col.positions), width).view(FT[width])
     exp = oracle_decode(col)
-    got = AL.decompress(AL.DeviceAlpColumn.from_host(col)).cpu().numpy()
-    assert np.array_equal(bits(got), bits(exp))
-    got = AL.decompress(AL.DeviceAlpColumn.from_host(AL.galp_convert(col))).cpu().numpy()
-    assert np.array_equal(bits(got), bits(exp))
+    d = AL.DeviceAlpColumn.from_host(col)
+    g = AL.DeviceAlpColumn.from_host(AL.galp_convert(col))
+    for variant in (0, A.VARIANT_HALF_LANE_STAGED, A.VARIANT_QUARTER_LANE_STAGED):
+        got = AL.decompress(d, variant=variant).cpu().numpy()
+        assert np.array_equal(bits(got), bits(exp)), f"variant {variant}"
+        got = AL.decompress(g, variant=variant).cpu().numpy()
+        assert np.array_equal(bits(got), bits(exp)), f"G-ALP variant {variant}"
+
+
+@pytest.mark.parametrize("width", [32, 64])
+def test_staged_variants_misaligned_output(width):
+    """The LDS-staged variants need a 16-byte aligned output; any other
+    output pointer takes the per-thread store path with the same result."""
+    rng = np.random.default_rng(79)
+    col = reference_style_column(rng, width, 7, rng.integers(0, width + 1, 7), 10)
+    exp = oracle_decode(col)
+    d = AL.DeviceAlpColumn.from_host(col)
+    dt = torch.float32 if width == 32 else torch.float64
+    for variant in (A.VARIANT_HALF_LANE_STAGED, A.VARIANT_QUARTER_LANE_STAGED):
+        buf = torch.full((col.n_vecs * 1024 + 1,), -1.0, dtype=dt, device="cuda")
+        out = buf[1:]
+        assert out.data_ptr() % 16 != 0
+        AL.decompress(d, out=out, variant=variant)
+        assert np.array_equal(bits(out.cpu().numpy()), bits(exp))
+        assert buf[0].item() == -1.0
 
 
 def c3_like(width, n_vecs, rng):

@@ -77,6 +77,25 @@ def test_decompress_mixed_widths_and_bp(width, vname, variant):
 
 
 @pytest.mark.parametrize("width", [32, 64])
+def test_staged_variants_misaligned_output(width):
+    """A staged variant with an output that is not 16-byte aligned takes the
+    per-thread store path; the words around the output stay untouched."""
+    rng = np.random.default_rng(21 + width)
+    n_vecs = 9
+    packed, bws, offs, bases = random_column(rng, n_vecs, width, rng.integers(0, width + 1, n_vecs))
+    col = gpu_column(packed, bws, offs, bases, width)
+    exp = oracle_decode(packed, bws, offs, bases, width)
+    dt = torch.uint32 if width == 32 else torch.uint64
+    for variant in (A.VARIANT_HALF_LANE_STAGED, A.VARIANT_QUARTER_LANE_STAGED):
+        buf = torch.zeros(n_vecs * 1024 + 2, dtype=dt, device="cuda")
+        out = buf[1:-1]
+        assert out.data_ptr() % 16 != 0
+        D.decompress(col, out=out, variant=variant)
+        assert np.array_equal(out.cpu().numpy(), exp)
+        assert int(buf[0].item()) == 0 and int(buf[-1].item()) == 0
+
+
+@pytest.mark.parametrize("width", [32, 64])
 def test_decompress_edges(width):
     rng = np.random.default_rng(3)
     # empty column
