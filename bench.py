@@ -12,8 +12,9 @@ Extra, untimed-by-the-driver fields report the other shapes: the 2^25-value
 uint32 width-13 column (decode and fused sum), the fused filter + sum sweep,
 and per-width roofline fractions.
 
-`roofline` is measured on the dominant kernel (the uint64 FFOR decode) with
-HIP events around each launch on the launch stream. `cpu_baseline` times the
+`roofline` is measured on the dominant kernel (the uint64 FFOR decode, the
+autotuned variant at each width) with HIP events around each launch on the
+launch stream. `cpu_baseline` times the
 reference's own CPU decoder (compiled from its sources into oracle/_ref) on a
 bounded sample of the same sweep, single-threaded.
 """
@@ -40,6 +41,8 @@ def parse():
     ap.add_argument("--variant", type=int, default=0, help="0 = autotuned default")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extras", action="store_true", help="only the timed sweep")
+    ap.add_argument("--extras", default="c1,filter,alp,batch,selector",
+                    help="comma list of the extra workloads to run")
     ap.add_argument("--out", default=None, help="also write the JSON line here")
     return ap.parse_args()
 
@@ -159,7 +162,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(),
-                     "kernel": "k_ffor_decompress<uint64_t,...> (average over w=0..64)",
+                     "kernel": "FFOR u64 decode, autotuned variant per width (average over "
+                               "w=0..64): " + variant_mix(A, [variants[i] for i in u64]),
                      "avg_launch_us": round(avg_ms * 1e3, 2),
                      "avg_algorithmic_bytes": int(avg_bytes)},
         "u32_sweep": {"avg_launch_us": round(avg_ms32 * 1e3, 2),
@@ -182,6 +186,16 @@ def main():
     if args.out:
         with open(args.out, "w") as f:
             f.write(line + "\n")
+
+
+def variant_mix(A, vs):
+    """'name xN, ...' of the decoder variants used, most used first."""
+    names = {v: k for k, v in A.VARIANTS.items()}
+    counts = {}
+    for v in vs:
+        counts[v] = counts.get(v, 0) + 1
+    return ", ".join(f"{names.get(v, v)} x{c}" for v, c in
+                     sorted(counts.items(), key=lambda kv: -kv[1]))
 
 
 TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
@@ -232,7 +246,27 @@ def extras(args, torch, A, D, E, dev, stream, out32):
     filter+sum sweep at 2^28 values."""
     import numpy as np
     res = {}
-    # 2^25 uint32 values 1e6 + U[0, 2^13), FOR base = vector min, width 13
+    which = set(args.extras.split(","))
+    r = torch.empty(2, dtype=torch.uint64, device=dev)
+    if "c1" in which:
+        res.update(c1_extras(args, torch, A, D, E, dev, stream, r))
+    if "filter" in which:
+        res["filter_sum_sweep"] = filter_sum_extras(args, torch, A, D, E, dev, stream, r)
+    torch.cuda.empty_cache()
+    if "alp" in which:
+        res.update(alp_extras(torch, dev, stream))
+    if "batch" in which:
+        res["c4_batch"] = batch_extras(torch, dev, stream)
+    if "selector" in which:
+        res["c5_selector"] = selector_extras(torch, dev)
+    return res
+
+
+def c1_extras(args, torch, A, D, E, dev, stream, r):
+    """2^25 uint32 values 1e6 + U[0, 2^13), FOR base = vector min, width 13:
+    decode to HBM and fused sum."""
+    import numpy as np
+    res = {}
     rng = np.random.default_rng(2025)
     vals = (1_000_000 + rng.integers(0, 1 << 13, 1 << 25)).astype(np.uint32)
     p, b, o, s = E.ffor_encode_uniform(vals, 13)
@@ -249,7 +283,6 @@ def extras(args, torch, A, D, E, dev, stream, out32):
     res["c1_u32_w13_2^25_decode"] = {"us": round(ms * 1e3, 2),
                                       "decompressed_GBps": round(nbytes / ms / 1e6, 1),
                                       "hbm_frac": round(algo / ms / 1e6 / HBM_PEAK_GBS, 4)}
-    r = torch.empty(2, dtype=torch.uint64, device=dev)
 
     def fsum():
         D.query_filter_sum(cols[k[0] % 4], A.PRED_ALL, result=r, stream=stream)
@@ -261,7 +294,11 @@ def extras(args, torch, A, D, E, dev, stream, out32):
                                    "hbm_frac": round((algo - nbytes) / ms / 1e6 / HBM_PEAK_GBS, 4),
                                    "sum_ok": got_sum == int(vals.astype(np.uint64).sum())}
     del cols
-    # fused filter (offset < 2^(w-1)) + sum over the sweep widths (2^28 values)
+    return res
+
+
+def filter_sum_extras(args, torch, A, D, E, dev, stream, r):
+    """Fused filter (offset < 2^(w-1)) + sum over sweep widths (2^28 values)."""
     nvals = 1 << args.values_log2
     n_vecs = nvals // 1024
     fs = []
@@ -276,12 +313,7 @@ def extras(args, torch, A, D, E, dev, stream, out32):
                        "decompressed_equiv_GBps": round(nvals * width / 8 / ms / 1e6, 1),
                        "hbm_frac": round(read / ms / 1e6 / HBM_PEAK_GBS, 4)})
             del col, pk, bb, oo, ss
-    res["filter_sum_sweep"] = fs
-    torch.cuda.empty_cache()
-    res.update(alp_extras(torch, dev, stream))
-    res["c4_batch"] = batch_extras(torch, dev, stream)
-    res["c5_selector"] = selector_extras(torch, dev)
-    return res
+    return fs
 
 
 def selector_extras(torch, dev):

@@ -316,6 +316,39 @@ int flsgpu_batch_run(flsgpu_batch_plan *plan, flsgpu_stream_t stream);
 int flsgpu_batch_graph_launch(flsgpu_batch_plan *plan, flsgpu_stream_t stream);
 int flsgpu_batch_plan_destroy(flsgpu_batch_plan *plan);
 
+/* ---- column files (.flsc): one encoded column on disk. Stands in for the
+ *      reference's byte storer (src/alp/storer.hpp:10-53) and its size model
+ *      (src/alp/alp-bindings.cu:19-43), which only append arrays to memory,
+ *      with a self-describing file: a 4 KiB header (magic "FLSGPUC1",
+ *      version, encoding, sizes, ALP-RD scalars, section table) and the
+ *      column's arrays at 256-byte aligned offsets. Loading validates every
+ *      size and offset a decoder derives from the metadata (bit widths,
+ *      vector and exception offsets, counts, G-ALP headers, positions, ALP
+ *      table indices), so a corrupt file fails with FLSGPU_EINVAL instead of
+ *      reaching a kernel. --------------------------------------------------- */
+typedef struct flsgpu_file_info {
+  int encoding;             /* enum flsgpu_encoding */
+  uint64_t n_values;
+  uint64_t n_vecs;
+  uint64_t payload_bytes;   /* bytes of the column's arrays (device block size) */
+  uint64_t file_bytes;
+} flsgpu_file_info;
+
+/* Writes a column whose descriptor (the struct named by `encoding` in enum
+ * flsgpu_encoding) holds HOST pointers. Only the used part of each array is
+ * stored (packed words up to the last vector's end, n_exceptions entries). */
+int flsgpu_file_write(const char *path, int encoding, const void *host_column);
+int flsgpu_file_stat(const char *path, flsgpu_file_info *info);
+/* Reads the column into one new device block (*d_block; release with
+ * flsgpu_free) and fills `column` (the descriptor type of `encoding`; the
+ * file must hold that encoding) with device pointers into it. Staged through
+ * two pinned buffers; returns after the copy has completed on `stream`. */
+int flsgpu_file_load(const char *path, int encoding, void *column, void **d_block,
+                     flsgpu_stream_t stream);
+/* The same into host memory (*h_block; release with flsgpu_file_free_host). */
+int flsgpu_file_read_host(const char *path, int encoding, void *column, void **h_block);
+void flsgpu_file_free_host(void *h_block);
+
 #ifdef __cplusplus
 }
 #endif

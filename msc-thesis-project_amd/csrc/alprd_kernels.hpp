@@ -93,7 +93,7 @@ __device__ __forceinline__ void alprd_decompress_unit(const AlprdArgs<T> &a, T *
   const uint64_t eo = a.exceptions_offsets[u.vec];
   const unsigned count = a.counts[u.vec];
   for (unsigned k = 0; k < count; ++k) {
-    const unsigned pos = a.positions[eo + k];
+    const unsigned pos = a.positions[eo + k] & 1023u;
     if (int(pos % L::N) != u.lane) continue;
     const UT right = extract_runtime<UT>(rin, int(a.right_bw), int(pos / L::N));
     const UT v = UT(UT(a.exceptions[eo + k]) << a.right_bw) | right;

@@ -64,13 +64,13 @@ def tune_ffor(values_log2: int = 24, reps: int = 10, device="cuda"):
     return out
 
 
-def tune_alp(reps: int = 10, device="cuda"):
+def tune_alp(values_log2: int = 24, reps: int = 10, device="cuda"):
     from . import alp as AL
     from . import workloads as WL
     out = {}
     rng = np.random.default_rng(7)
     for width in (32, 64):
-        x = WL.alp_decimal_values(width, 1 << 22, rng)
+        x = WL.alp_decimal_values(width, 1 << values_log2, rng)
         col = AL.alp_encode(x)
         for name, c in ((f"alp_f{width}", col), (f"alp_ext_f{width}", AL.galp_convert(col))):
             d = AL.DeviceAlpColumn.from_host(c, device)
@@ -115,7 +115,7 @@ def main():
     args = ap.parse_args()
     A.check(A.lib().flsgpu_init(0))
     table = tune_ffor(args.values_log2)
-    table.update(tune_alp())
+    table.update(tune_alp(args.values_log2))
     # the column default is the variant that wins most widths
     for enc in ("ffor_u32", "ffor_u64"):
         vals = list(table[enc].values())
