@@ -319,6 +319,41 @@ __global__ __launch_bounds__(BLOCK) void k_alp_decompress_staged(AlpArgs<T> a, T
   T *lds = s_stage[threadIdx.x / WAVE];
   const int t = int(threadIdx.x % WAVE), vv = t / L::N;
   const int j0 = u.part * Sh::NJ;
+  const uint64_t group = wave / S;
+  // The first PF * 64 exceptions of each vector are loaded before the decode
+  // so that their latency overlaps it; longer lists finish in a loop after.
+  constexpr int PF = PATCH ? (sizeof(T) == 4 ? 3 : 2) : 1;
+  uint64_t eo[L::VPW];
+  unsigned count[L::VPW];
+  unsigned pf_pos[L::VPW][PF];
+  T pf_val[L::VPW][PF];
+  if constexpr (PATCH) {
+#pragma unroll
+    for (int v2 = 0; v2 < L::VPW; ++v2) {
+      const uint64_t vec = group * L::VPW + v2;
+      count[v2] = 0;
+      eo[v2] = 0;
+      if (vec < a.ffor.n_vecs) {
+        eo[v2] = a.exceptions_offsets[vec];
+        if constexpr (LAYOUT == ALP_GALP) {
+          // lanes are concatenated in order: the last lane's offset + count
+          const unsigned oc = a.offsets_counts[vec * L::N + (L::N - 1)];
+          count[v2] = (oc & 0x3FFu) + (oc >> 10);
+        } else {
+          count[v2] = a.counts[vec];
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < PF; ++p) {
+        const unsigned k = unsigned(t + p * WAVE);
+        pf_pos[v2][p] = 0xFFFFu;  // never inside a slot range
+        if (k < count[v2]) {
+          pf_pos[v2][p] = a.positions[eo[v2] + k] & 1023u;
+          pf_val[v2][p] = a.exceptions[eo[v2] + k];
+        }
+      }
+    }
+  }
   if (u.vec < a.ffor.n_vecs) {
     AlpStagePolicy<T> pol{a.ffor.packed + a.ffor.vector_offsets[u.vec] + u.lane,
                           lds + vv * Sh::CHUNK + u.lane, a.ffor.bases[u.vec],
@@ -330,23 +365,17 @@ __global__ __launch_bounds__(BLOCK) void k_alp_decompress_staged(AlpArgs<T> a, T
   }
   if constexpr (PATCH) {
     wave_lds_sync();
-    const uint64_t group = wave / S;
     const unsigned lo = unsigned(L::N * j0), hi = unsigned(L::N * (j0 + Sh::NJ));
+#pragma unroll
     for (int v2 = 0; v2 < L::VPW; ++v2) {
-      const uint64_t vec = group * L::VPW + v2;
-      if (vec >= a.ffor.n_vecs) break;
-      const uint64_t eo = a.exceptions_offsets[vec];
-      unsigned count;
-      if constexpr (LAYOUT == ALP_GALP) {
-        // lanes are concatenated in order: the last lane's offset + count
-        const unsigned oc = a.offsets_counts[vec * L::N + (L::N - 1)];
-        count = (oc & 0x3FFu) + (oc >> 10);
-      } else {
-        count = a.counts[vec];
+#pragma unroll
+      for (int p = 0; p < PF; ++p) {
+        const unsigned pos = pf_pos[v2][p];
+        if (pos >= lo && pos < hi) lds[v2 * Sh::CHUNK + (pos - lo)] = pf_val[v2][p];
       }
-      for (unsigned k = unsigned(t); k < count; k += WAVE) {
-        const unsigned pos = a.positions[eo + k] & 1023u;
-        if (pos >= lo && pos < hi) lds[v2 * Sh::CHUNK + (pos - lo)] = a.exceptions[eo + k];
+      for (unsigned k = unsigned(t + PF * WAVE); k < count[v2]; k += WAVE) {
+        const unsigned pos = a.positions[eo[v2] + k] & 1023u;
+        if (pos >= lo && pos < hi) lds[v2 * Sh::CHUNK + (pos - lo)] = a.exceptions[eo[v2] + k];
       }
     }
   }
