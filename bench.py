@@ -228,6 +228,36 @@ def copy_bandwidth(torch, dev, stream):
     return 2 * (1 << 31) / ms / 1e6
 
 
+class CacheFlush:
+    """Evicts the 256 MB Infinity Cache and the L2s before a timed launch by
+    reading a 1 GiB scratch buffer, so a launch repeated on one column reads
+    it from HBM, as a first launch would. A read leaves clean lines behind; a
+    write would leave dirty ones whose write-back lands in the timed launch."""
+
+    def __init__(self, torch, dev):
+        self.torch = torch
+        self.buf = torch.ones(1 << 27, dtype=torch.int64, device=dev)
+
+    def __call__(self):
+        self.torch.sum(self.buf)
+
+
+def cold_launch_ms(torch, stream, fn, reps, flush):
+    """Mean duration of `reps` launches of fn, each after a cache flush and
+    bracketed by its own events on `stream`."""
+    fn()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(reps)]
+    for a, b in evs:
+        flush()
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    return sum(a.elapsed_time(b) for a, b in evs) / reps
+
+
 def time_launches(torch, stream, fn, reps):
     a = torch.cuda.Event(enable_timing=True)
     b = torch.cuda.Event(enable_timing=True)
@@ -248,13 +278,15 @@ def extras(args, torch, A, D, E, dev, stream, out32):
     res = {}
     which = set(args.extras.split(","))
     r = torch.empty(2, dtype=torch.uint64, device=dev)
+    flush = CacheFlush(torch, dev)
     if "c1" in which:
-        res.update(c1_extras(args, torch, A, D, E, dev, stream, r))
+        res.update(c1_extras(args, torch, A, D, E, dev, stream, r, flush))
     if "filter" in which:
-        res["filter_sum_sweep"] = filter_sum_extras(args, torch, A, D, E, dev, stream, r)
+        res["filter_sum_sweep"] = filter_sum_extras(args, torch, A, D, E, dev, stream, r, flush)
     torch.cuda.empty_cache()
     if "alp" in which:
-        res.update(alp_extras(torch, dev, stream))
+        res.update(alp_extras(torch, dev, stream, flush))
+    del flush
     if "batch" in which:
         res["c4_batch"] = batch_extras(torch, dev, stream)
     if "selector" in which:
@@ -262,42 +294,36 @@ def extras(args, torch, A, D, E, dev, stream, out32):
     return res
 
 
-def c1_extras(args, torch, A, D, E, dev, stream, r):
+def c1_extras(args, torch, A, D, E, dev, stream, r, flush):
     """2^25 uint32 values 1e6 + U[0, 2^13), FOR base = vector min, width 13:
-    decode to HBM and fused sum."""
+    decode to HBM and fused sum; every timed launch starts with cold caches."""
     import numpy as np
     res = {}
     rng = np.random.default_rng(2025)
     vals = (1_000_000 + rng.integers(0, 1 << 13, 1 << 25)).astype(np.uint32)
     p, b, o, s = E.ffor_encode_uniform(vals, 13)
-    cols = [D.DeviceFFORColumn.from_host(p, b, o, s, len(vals), dev) for _ in range(4)]
+    col = D.DeviceFFORColumn.from_host(p, b, o, s, len(vals), dev)
     out = torch.empty(len(vals), dtype=torch.uint32, device=dev)
-    k = [0]
-
-    def dec():
-        D.decompress(cols[k[0] % 4], out, variant=args.variant, stream=stream)
-        k[0] += 1
-    ms = time_launches(torch, stream, dec, 50)
+    ms = cold_launch_ms(torch, stream, lambda: D.decompress(col, out, variant=args.variant,
+                                                            stream=stream), 20, flush)
     nbytes = len(vals) * 4
     algo = p.nbytes + b.nbytes + o.nbytes + s.nbytes + nbytes
     res["c1_u32_w13_2^25_decode"] = {"us": round(ms * 1e3, 2),
                                       "decompressed_GBps": round(nbytes / ms / 1e6, 1),
                                       "hbm_frac": round(algo / ms / 1e6 / HBM_PEAK_GBS, 4)}
 
-    def fsum():
-        D.query_filter_sum(cols[k[0] % 4], A.PRED_ALL, result=r, stream=stream)
-        k[0] += 1
-    ms = time_launches(torch, stream, fsum, 50)
+    ms = cold_launch_ms(torch, stream, lambda: D.query_filter_sum(col, A.PRED_ALL, result=r,
+                                                                  stream=stream), 20, flush)
     got_sum = int(r.cpu().numpy()[1])
     res["c1_u32_w13_2^25_sum"] = {"us": round(ms * 1e3, 2),
                                    "decompressed_equiv_GBps": round(nbytes / ms / 1e6, 1),
                                    "hbm_frac": round((algo - nbytes) / ms / 1e6 / HBM_PEAK_GBS, 4),
                                    "sum_ok": got_sum == int(vals.astype(np.uint64).sum())}
-    del cols
+    del col
     return res
 
 
-def filter_sum_extras(args, torch, A, D, E, dev, stream, r):
+def filter_sum_extras(args, torch, A, D, E, dev, stream, r, flush):
     """Fused filter (offset < 2^(w-1)) + sum over sweep widths (2^28 values)."""
     nvals = 1 << args.values_log2
     n_vecs = nvals // 1024
@@ -306,8 +332,8 @@ def filter_sum_extras(args, torch, A, D, E, dev, stream, r):
         for bw in bws:
             pk, bb, oo, ss = E.synthetic_ffor_column_device(n_vecs, bw, width, seed=7 + bw, device=dev)
             col = D.DeviceFFORColumn(nvals, pk, bb, oo, ss, width)
-            ms = time_launches(torch, stream, lambda: D.query_filter_sum(
-                col, A.PRED_OFFSET_LOWER_HALF, result=r, stream=stream), 10)
+            ms = cold_launch_ms(torch, stream, lambda: D.query_filter_sum(
+                col, A.PRED_OFFSET_LOWER_HALF, result=r, stream=stream), 5, flush)
             read = pk.numel() * pk.element_size() + n_vecs * (1 + 8 + width // 8)
             fs.append({"dtype": f"u{width}", "bw": bw, "us": round(ms * 1e3, 1),
                        "decompressed_equiv_GBps": round(nvals * width / 8 / ms / 1e6, 1),
@@ -345,9 +371,12 @@ def batch_extras(torch, dev, stream):
     plan = B.BatchPlan(cols, outs)
     dec_bytes = sum(s.values.nbytes for s in specs)
     reps = 10
-    t_plain = time_launches(torch, stream, lambda: B.decode_each(cols, outs, stream=stream), reps)
-    t_batch = time_launches(torch, stream, lambda: plan.run(stream), reps)
-    t_graph = time_launches(torch, stream, lambda: plan.graph_launch(stream), reps)
+    flush = CacheFlush(torch, dev)
+    t_plain = cold_launch_ms(torch, stream, lambda: B.decode_each(cols, outs, stream=stream), reps,
+                             flush)
+    t_batch = cold_launch_ms(torch, stream, lambda: plan.run(stream), reps, flush)
+    t_graph = cold_launch_ms(torch, stream, lambda: plan.graph_launch(stream), reps, flush)
+    del flush
     ok = True
     for s, o in zip(specs, outs):
         v = s.values.view(np.int32 if s.values.itemsize == 4 else np.int64)
@@ -391,7 +420,7 @@ def c3_values(np, width, rng):
     return x
 
 
-def alp_extras(torch, dev, stream):
+def alp_extras(torch, dev, stream, flush):
     import numpy as np
     from flsgpu import alp as AL
     res = {}
@@ -407,7 +436,8 @@ def alp_extras(torch, dev, stream):
         for name, c in (("alp", col), ("galp", galp)):
             d = AL.DeviceAlpColumn.from_host(c, dev)
             out = torch.empty(len(x), dtype=torch.float64 if width == 64 else torch.float32, device=dev)
-            ms = time_launches(torch, stream, lambda: AL.decompress(d, out, stream=stream), 20)
+            ms = cold_launch_ms(torch, stream, lambda: AL.decompress(d, out, stream=stream), 10,
+                                flush)
             ok = bool(torch.equal(out.view(ref.dtype), ref))
             read = (c.packed.nbytes + c.bit_widths.nbytes + c.vector_offsets.nbytes + c.bases.nbytes
                     + c.factor_indices.nbytes * 2 + c.exceptions_offsets.nbytes + c.exceptions.nbytes

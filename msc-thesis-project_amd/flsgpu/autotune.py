@@ -5,7 +5,11 @@ current GPU and keeps the fastest, replacing the reference's offline
 pushes it into the library (flsgpu_set_tuning) so FLSGPU_VARIANT_AUTO picks it
 up, and variant_for() serves hosts that know a column's bit width.
 
-Run:  python -m flsgpu.autotune --out <table.json>
+Run:  python -m flsgpu.autotune --out <table.json> [--values-log2 28]
+
+Every launch is timed alone after a cache flush, at the column size given
+(default 2^28 values, the size of the bit-width sweep): smaller columns fit in
+the 256 MB Infinity Cache and rank the variants differently.
 """
 from __future__ import annotations
 
@@ -24,17 +28,43 @@ ENC = {"ffor_u32": 1, "ffor_u64": 3, "alp_f32": 4, "alp_f64": 5, "alp_ext_f32": 
 _table_cache: dict | None = None
 
 
+class _Flush:
+    """Reads a 1 GiB scratch buffer to evict the 256 MB Infinity Cache and
+    the L2s, so every timed launch reads its column from HBM (a column that
+    fits in the cache would otherwise favour latency-bound variants). A read
+    leaves clean lines; a write would leave dirty lines whose write-back
+    would be charged to the timed launch."""
+
+    def __init__(self, device="cuda"):
+        self.buf = torch.ones(1 << 27, dtype=torch.int64, device=device)
+
+    def __call__(self):
+        torch.sum(self.buf)
+
+
+_flush = None
+
+
 def _time(fn, reps=10):
+    """Mean duration of `reps` launches, each after a cache flush and
+    bracketed by its own events."""
+    global _flush
+    if _flush is None:
+        _flush = _Flush()
     fn()
     torch.cuda.synchronize()
-    a = torch.cuda.Event(enable_timing=True)
-    b = torch.cuda.Event(enable_timing=True)
-    a.record()
-    for _ in range(reps):
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(reps)]
+    for a, b in evs:
+        _flush()
+        a.record()
         fn()
-    b.record()
+        b.record()
     torch.cuda.synchronize()
-    return a.elapsed_time(b) / reps
+    return sum(a.elapsed_time(b) for a, b in evs) / reps
+
+
+TIMES: dict = {}  # (encoding, bit width) -> {variant name: ms}, filled by the tuners
 
 
 def tune_ffor(values_log2: int = 24, reps: int = 10, device="cuda"):
@@ -57,6 +87,7 @@ def tune_ffor(values_log2: int = 24, reps: int = 10, device="cuda"):
                     D.decompress(cols[k[0] % 2], res, variant=v)
                     k[0] += 1
                 ms = _time(run, reps)
+                TIMES.setdefault(f"{key}/{bw}", {})[name] = round(ms, 4)
                 if best is None or ms < best[1]:
                     best = (v, ms)
             table[str(bw)] = best[0]
@@ -76,8 +107,9 @@ def tune_alp(values_log2: int = 24, reps: int = 10, device="cuda"):
             d = AL.DeviceAlpColumn.from_host(c, device)
             res = torch.empty(len(x), dtype=torch.float32 if width == 32 else torch.float64, device=device)
             best = None
-            for v in A.VARIANTS.values():
+            for vname, v in A.VARIANTS.items():
                 ms = _time(lambda: AL.decompress(d, res, variant=v), reps)
+                TIMES.setdefault(f"{name}/255", {})[vname] = round(ms, 4)
                 if best is None or ms < best[1]:
                     best = (v, ms)
             out[name] = {"255": best[0]}
@@ -111,17 +143,21 @@ def variant_for(encoding: str, bit_width: int) -> int:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_TABLE)
-    ap.add_argument("--values-log2", type=int, default=24)
+    ap.add_argument("--values-log2", type=int, default=28)
     args = ap.parse_args()
     A.check(A.lib().flsgpu_init(0))
     table = tune_ffor(args.values_log2)
-    table.update(tune_alp(args.values_log2))
+    # ALP columns: the C3 size (2^26 doubles / floats); encoding 2^28 values
+    # on the host would dominate the run
+    table.update(tune_alp(min(args.values_log2, 26)))
     # the column default is the variant that wins most widths
     for enc in ("ffor_u32", "ffor_u64"):
         vals = list(table[enc].values())
         table[enc]["255"] = max(set(vals), key=vals.count)
     with open(args.out, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
+    with open(os.path.splitext(args.out)[0] + "_times.json", "w") as f:
+        json.dump({"values_log2": args.values_log2, "ms": TIMES}, f, indent=1, sort_keys=True)
     print(json.dumps(table))
 
 

@@ -77,6 +77,35 @@ def test_decompress_mixed_widths_and_bp(width, vname, variant):
 
 
 @pytest.mark.parametrize("width", [32, 64])
+def test_filter_sum_mixed_widths(width):
+    """Widths that differ between vectors (the fused kernel's per-unit path
+    next to its batched same-width path) and a vector with a width above W,
+    which contributes no rows."""
+    rng = np.random.default_rng(14)
+    m64 = (1 << 64) - 1
+    n_vecs = 67
+    widths = np.full(n_vecs, 11, np.int64)
+    widths[20:45] = rng.integers(0, width + 1, 25)  # mixed block in the middle
+    packed, bws, offs, bases = random_column(rng, n_vecs, width, widths)
+    bases = rng.integers(0, 1 << 20, n_vecs).astype(UT[width])
+    bad = bws.copy()
+    bad[30] = width + 1
+    col = gpu_column(packed, bad, offs, bases, width)
+    dec = oracle_decode(packed, bws, offs, bases, width).reshape(n_vecs, 1024)
+    keep = np.ones(n_vecs, bool)
+    keep[30] = False
+    off = dec - bases[:, None]
+    half = np.array([(1 << (int(b) - 1)) if b > 0 else 0 for b in bws], dtype=np.uint64)
+    sel = (off.astype(np.uint64) < half[:, None]) & keep[:, None]
+    r = D.query_filter_sum(col, A.PRED_OFFSET_LOWER_HALF).cpu().numpy()
+    assert int(r[0]) == int(sel.sum())
+    assert int(r[1]) == sum(int(x) for x in dec[sel]) & m64
+    r = D.query_filter_sum(col, A.PRED_ALL).cpu().numpy()
+    assert int(r[0]) == 1024 * int(keep.sum())
+    assert int(r[1]) == sum(int(x) for x in dec[keep].reshape(-1)) & m64
+
+
+@pytest.mark.parametrize("width", [32, 64])
 def test_staged_variants_misaligned_output(width):
     """A staged variant with an output that is not 16-byte aligned takes the
     per-thread store path; the words around the output stay untouched."""
