@@ -322,7 +322,7 @@ __global__ __launch_bounds__(BLOCK) void k_alp_decompress_staged(AlpArgs<T> a, T
   const uint64_t group = wave / S;
   // The first PF * 64 exceptions of each vector are loaded before the decode
   // so that their latency overlaps it; longer lists finish in a loop after.
-  constexpr int PF = PATCH ? (sizeof(T) == 4 ? 3 : 2) : 1;
+  constexpr int PF = PATCH ? (sizeof(T) == 4 ? 3 : 1) : 1;
   uint64_t eo[L::VPW];
   unsigned count[L::VPW];
   unsigned pf_pos[L::VPW][PF];
