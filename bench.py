@@ -376,7 +376,18 @@ def batch_extras(torch, dev, stream):
                              flush)
     t_batch = cold_launch_ms(torch, stream, lambda: plan.run(stream), reps, flush)
     t_graph = cold_launch_ms(torch, stream, lambda: plan.graph_launch(stream), reps, flush)
+    # the plain launches per kind, and the bytes a decode must move
+    per_kind = {}
+    for kind in sorted({s.kind for s in specs}):
+        idx = [i for i, s in enumerate(specs) if s.kind == kind]
+        kc, ko = [cols[i] for i in idx], [outs[i] for i in idx]
+        t = cold_launch_ms(torch, stream, lambda: B.decode_each(kc, ko, stream=stream), 5, flush)
+        per_kind[kind] = {"plain_us": round(t * 1e3, 1),
+                          "values": int(sum(len(specs[i].values) for i in idx))}
     del flush
+    read_bytes = sum(t.numel() * t.element_size() for c in cols for t in vars(c).values()
+                     if isinstance(t, torch.Tensor))
+    algo = read_bytes + dec_bytes
     ok = True
     for s, o in zip(specs, outs):
         v = s.values.view(np.int32 if s.values.itemsize == 4 else np.int64)
@@ -393,7 +404,10 @@ def batch_extras(torch, dev, stream):
            "batched_GBps": round(dec_bytes / t_batch / 1e6, 1),
            "graph_GBps": round(dec_bytes / t_graph / 1e6, 1),
            "speedup_batched_vs_plain": round(t_plain / t_batch, 3),
-           "speedup_graph_vs_plain": round(t_plain / t_graph, 3)}
+           "speedup_graph_vs_plain": round(t_plain / t_graph, 3),
+           "algorithmic_bytes": int(algo),
+           "batched_hbm_frac": round(algo / t_batch / 1e6 / HBM_PEAK_GBS, 4),
+           "plain_per_kind": per_kind}
     plan.close()
     del cols, outs
     torch.cuda.empty_cache()
@@ -452,6 +466,26 @@ def alp_extras(torch, dev, stream, flush):
             del d, out
         del ref
         torch.cuda.empty_cache()
+    # ALP-RD (the C4 batch's third kind): 2^26 doubles uniform in [0, 1)
+    x = rng.uniform(0.0, 1.0, 1 << 26)
+    t0 = time.perf_counter()
+    rd = AL.alprd_encode(x)
+    enc_s = time.perf_counter() - t0
+    d = AL.DeviceAlprdColumn.from_host(rd, dev)
+    out = torch.empty(len(x), dtype=torch.float64, device=dev)
+    ms = cold_launch_ms(torch, stream, lambda: AL.alprd_decompress(d, out, stream=stream), 10, flush)
+    ok = bool(torch.equal(out.view(torch.int64), torch.from_numpy(x.view(np.int64)).to(dev)))
+    read = sum(a.nbytes for a in (rd.right_packed, rd.left_packed, rd.exceptions_offsets,
+                                  rd.exceptions, rd.positions, rd.counts))
+    res["alprd_f64_2^26"] = {"us": round(ms * 1e3, 1), "decompressed_GBps": round(x.nbytes / ms / 1e6, 1),
+                             "hbm_frac": round((read + x.nbytes) / ms / 1e6 / HBM_PEAK_GBS, 4),
+                             "bit_exact": ok, "right_bit_width": int(rd.right_bw),
+                             "left_bit_width": int(rd.left_bw),
+                             "exceptions_per_vector": round(len(rd.exceptions) / rd.n_vecs, 2),
+                             "compression_ratio": round(x.nbytes / rd.compressed_bytes, 3),
+                             "host_encode_s": round(enc_s, 2)}
+    del d, out
+    torch.cuda.empty_cache()
     return res
 
 

@@ -59,7 +59,11 @@ enum flsgpu_variant {
    * variant of the same split runs). */
   FLSGPU_VARIANT_HALF_LANE_STAGED = 7,
   FLSGPU_VARIANT_QUARTER_LANE_STAGED = 8,
-  FLSGPU_VARIANT_COUNT = 9
+  /* staged, eight threads per lane (64-bit FFOR / ALP / ALP-RD and 32-bit
+   * FFOR; 32-bit ALP and ALP-RD use their HALF_LANE_STAGED split): a smaller
+   * LDS stage per wave, so more waves in flight per CU */
+  FLSGPU_VARIANT_EIGHTH_LANE_STAGED = 9,
+  FLSGPU_VARIANT_COUNT = 10
 };
 
 /* Reference enums::Patcher values (src/engine/enums.cuh:43-53). */

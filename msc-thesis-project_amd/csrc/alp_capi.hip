@@ -81,6 +81,11 @@ static int alp_shape(int variant, int encoding, int *S, bool *nt, bool *staged) 
       *nt = true;
       *staged = true;
       return FLSGPU_OK;
+    case FLSGPU_VARIANT_EIGHTH_LANE_STAGED:
+      *S = wide ? 8 : 2;
+      *nt = true;
+      *staged = true;
+      return FLSGPU_OK;
     case FLSGPU_VARIANT_LANE: *S = 1; *nt = false; return FLSGPU_OK;
     case FLSGPU_VARIANT_LANE_NT: *S = 1; *nt = true; return FLSGPU_OK;
     case FLSGPU_VARIANT_HALF_LANE: *S = 2; *nt = false; return FLSGPU_OK;
@@ -122,9 +127,12 @@ static int alp_decompress(const flsgpu_alp_column *c, T *out, int layout, int pa
   if (staged && (reinterpret_cast<uintptr_t>(out) % 16) == 0) {
     if constexpr (sizeof(T) == 4) {
       if (S == 2) return decompress_staged<T, 2>(a, out, layout, patch, st);
+    } else {
+      if (S == 8) return decompress_staged<T, 8>(a, out, layout, patch, st);
     }
     return decompress_staged<T, 4>(a, out, layout, patch, st);
   }
+  if (S == 8) S = 4;  // unaligned output: the NT variant of the nearest split
   if (S == 1) return nt ? decompress_s<T, 1, true>(a, out, layout, patch, st)
                         : decompress_s<T, 1, false>(a, out, layout, patch, st);
   if (S == 2) return nt ? decompress_s<T, 2, true>(a, out, layout, patch, st)

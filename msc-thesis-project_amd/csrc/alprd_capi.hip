@@ -36,13 +36,24 @@ static int alprd_decompress(const flsgpu_alprd_column *c, T *out, int variant, h
     variant = flsgpu_get_tuning(W == 32 ? FLSGPU_ENC_ALPRD_F32 : FLSGPU_ENC_ALPRD_F64, 255);
   if (variant < FLSGPU_VARIANT_LANE || variant >= FLSGPU_VARIANT_COUNT)
     return set_error(FLSGPU_EUNSUPPORTED, "unknown variant %d", variant);
-  // The non-temporal and staged variants map to non-temporal stores; lane
-  // splitting and staging are not provided for ALP-RD (a lane's four uint16
-  // left lanes are loaded once).
-  const bool nt = variant == FLSGPU_VARIANT_LANE_NT || variant == FLSGPU_VARIANT_HALF_LANE_NT ||
-                  variant == FLSGPU_VARIANT_QUARTER_LANE_NT ||
-                  variant == FLSGPU_VARIANT_HALF_LANE_STAGED ||
-                  variant == FLSGPU_VARIANT_QUARTER_LANE_STAGED;
+  // The staged variants (S = 2 for float; 4, or 8 for EIGHTH, for double) decode through
+  // LDS when the output is 16-byte aligned; otherwise they and the other
+  // non-temporal variants use the one-thread-per-lane kernel with
+  // non-temporal stores. Lane splitting without staging is not provided for
+  // ALP-RD (each thread loads its lane's uint16 left lanes once).
+  const bool staged = variant == FLSGPU_VARIANT_HALF_LANE_STAGED ||
+                      variant == FLSGPU_VARIANT_QUARTER_LANE_STAGED ||
+                      variant == FLSGPU_VARIANT_EIGHTH_LANE_STAGED;
+  if (staged && (reinterpret_cast<uintptr_t>(out) % 16) == 0 && a.n_vecs) {
+    if constexpr (W == 64) {
+      if (variant == FLSGPU_VARIANT_EIGHTH_LANE_STAGED)
+        return AlprdDecompressStaged<T, 8>::launch(a, out, st);
+    }
+    return AlprdDecompressStaged<T, W == 32 ? 2 : 4>::launch(a, out, st);
+  }
+  const bool nt = staged || variant == FLSGPU_VARIANT_LANE_NT ||
+                  variant == FLSGPU_VARIANT_HALF_LANE_NT ||
+                  variant == FLSGPU_VARIANT_QUARTER_LANE_NT;
   return nt ? AlprdDecompress<T, true>::launch(a, out, st) : AlprdDecompress<T, false>::launch(a, out, st);
 }
 

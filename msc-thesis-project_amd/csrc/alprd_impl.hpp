@@ -24,4 +24,11 @@ struct AlprdDecompress {
   static int launch(const AlprdArgs<T> &a, T *out, hipStream_t stream);
 };
 
+// Decode through LDS in S slot parts per lane, exceptions applied in LDS,
+// 16-byte non-temporal stores (out must be 16-byte aligned).
+template <typename T, int S>
+struct AlprdDecompressStaged {
+  static int launch(const AlprdArgs<T> &a, T *out, hipStream_t stream);
+};
+
 }  // namespace flsgpu_hip

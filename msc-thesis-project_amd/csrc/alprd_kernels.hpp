@@ -15,6 +15,8 @@
 
 #include "alprd_impl.hpp"
 #include "kernels.hpp"
+#include "stage.hpp"
+#include "unpack.hpp"
 
 namespace flsgpu_hip {
 
@@ -107,6 +109,134 @@ __global__ __launch_bounds__(BLOCK) void k_alprd_decompress(AlprdArgs<T> a, T *_
   alprd_fill_dictionary<T>(a, dshift);
   __syncthreads();
   alprd_decompress_unit<T, NT>(a, out, (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE, dshift);
+}
+
+template <typename T>
+struct RdStagePolicy {
+  using UT = typename uint_of<T>::type;
+  static constexpr int NL = lanes<UT>::N;
+  static constexpr int Q = 64 / NL;
+  const UT *rin;
+  T *lds;  // this lane's slot J0 in the wave's stage
+  const UT *dshift;
+  uint64_t lstream[Q];
+  unsigned lbw;
+  template <int BW, int J0, int NJ>
+  __device__ __forceinline__ void run() {
+    const uint64_t lmask = (uint64_t(1) << lbw) - 1;
+    unpack_slots<UT, BW, J0, NJ>(rin, [&](int j, UT r) {
+      const unsigned idx = unsigned(lstream[j % Q] >> ((j / Q) * lbw)) & unsigned(lmask);
+      lds[(j - J0) * NL] = __builtin_bit_cast(T, UT(dshift[idx] | r));
+    });
+  }
+};
+
+// Right width (W-16 .. W-1, wave-uniform) and slot part.
+template <typename UT, int S, class Policy>
+__device__ __forceinline__ bool dispatch_right_bw_part(Policy &pol, int bw, int part) {
+  constexpr int W = lanes<UT>::W;
+  const int b = __builtin_amdgcn_readfirstlane(bw);
+  switch (b - (W - 16)) {
+#define FLSGPU_RD_CASE(K) \
+  case K: dispatch_part<UT, S, W - 16 + K>(pol, part); return true;
+    FLSGPU_RD_CASE(0) FLSGPU_RD_CASE(1) FLSGPU_RD_CASE(2) FLSGPU_RD_CASE(3)
+    FLSGPU_RD_CASE(4) FLSGPU_RD_CASE(5) FLSGPU_RD_CASE(6) FLSGPU_RD_CASE(7)
+    FLSGPU_RD_CASE(8) FLSGPU_RD_CASE(9) FLSGPU_RD_CASE(10) FLSGPU_RD_CASE(11)
+    FLSGPU_RD_CASE(12) FLSGPU_RD_CASE(13) FLSGPU_RD_CASE(14) FLSGPU_RD_CASE(15)
+#undef FLSGPU_RD_CASE
+    default: return false;
+  }
+}
+
+// Staged ALP-RD decode of one wave-unit: each thread decodes its lane's
+// slot part into LDS; the wave then walks each vector's exception list with
+// coalesced loads (issued before the decode) and rebuilds the values at the
+// positions in its slot range from the staged right part and the stored left
+// part; then 16-byte stores.
+template <typename T, int S>
+__global__ __launch_bounds__(BLOCK) void k_alprd_decompress_staged(AlprdArgs<T> a,
+                                                                   T *__restrict out) {
+  using UT = typename uint_of<T>::type;
+  using L = lanes<UT>;
+  using Sh = StageShape<UT, S>;
+  constexpr int Q = 64 / L::N;
+  __shared__ UT dshift[8];
+  __shared__ __align__(16) T s_stage[BLOCK / WAVE][L::VPW * Sh::CHUNK];
+  alprd_fill_dictionary<T>(a, dshift);
+  __syncthreads();
+  const uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE;
+  const UnitMap<UT, S> u = map_unit<UT, S>(wave);
+  T *lds = s_stage[threadIdx.x / WAVE];
+  const int t = int(threadIdx.x % WAVE), vv = t / L::N;
+  const int j0 = u.part * Sh::NJ;
+  const uint64_t group = wave / S;
+  // the first 64 exceptions of each vector, loaded before the decode
+  uint64_t eo[L::VPW];
+  unsigned count[L::VPW], pf_pos[L::VPW];
+  uint16_t pf_left[L::VPW];
+#pragma unroll
+  for (int v2 = 0; v2 < L::VPW; ++v2) {
+    const uint64_t vec = group * L::VPW + v2;
+    count[v2] = 0;
+    eo[v2] = 0;
+    pf_pos[v2] = 0xFFFFu;  // never inside a slot range
+    if (vec < a.n_vecs) {
+      eo[v2] = a.exceptions_offsets[vec];
+      count[v2] = a.counts[vec];
+      if (unsigned(t) < count[v2]) {
+        pf_pos[v2] = a.positions[eo[v2] + t] & 1023u;
+        pf_left[v2] = a.exceptions[eo[v2] + t];
+      }
+    }
+  }
+  if (u.vec < a.n_vecs) {
+    RdStagePolicy<T> pol;
+    pol.rin = a.right + u.vec * uint64_t(a.right_bw) * L::N + u.lane;
+    pol.lds = lds + vv * Sh::CHUNK + u.lane;
+    pol.dshift = dshift;
+    pol.lbw = a.left_bw;
+    const uint16_t *lin = a.left + u.vec * uint64_t(a.left_bw) * 64;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      uint64_t s = 0;
+      for (unsigned r = 0; r < a.left_bw; ++r)
+        s |= uint64_t(lin[u.lane + L::N * q + 64 * r]) << (16 * r);
+      pol.lstream[q] = s;
+    }
+    dispatch_right_bw_part<UT, S>(pol, int(a.right_bw), u.part);  // widths checked on the host
+  }
+  wave_lds_sync();
+  const unsigned lo = unsigned(L::N * j0), hi = unsigned(L::N * (j0 + Sh::NJ));
+  const UT rmask = UT((UT(1) << a.right_bw) - 1);
+#pragma unroll
+  for (int v2 = 0; v2 < L::VPW; ++v2) {
+    if (pf_pos[v2] >= lo && pf_pos[v2] < hi) {
+      T &x = lds[v2 * Sh::CHUNK + (pf_pos[v2] - lo)];
+      x = __builtin_bit_cast(T, UT((UT(pf_left[v2]) << a.right_bw) |
+                                   (__builtin_bit_cast(UT, x) & rmask)));
+    }
+    for (unsigned k = unsigned(t + WAVE); k < count[v2]; k += WAVE) {
+      const unsigned pos = a.positions[eo[v2] + k] & 1023u;
+      if (pos >= lo && pos < hi) {
+        T &x = lds[v2 * Sh::CHUNK + (pos - lo)];
+        x = __builtin_bit_cast(T, UT((UT(a.exceptions[eo[v2] + k]) << a.right_bw) |
+                                     (__builtin_bit_cast(UT, x) & rmask)));
+      }
+    }
+  }
+  wave_lds_sync();
+  store_staged<T, S>(lds, out, group, u.part, a.n_vecs);
+}
+
+template <typename T, int S>
+int AlprdDecompressStaged<T, S>::launch(const AlprdArgs<T> &a, T *out, hipStream_t stream) {
+  using UT = typename uint_of<T>::type;
+  if (a.n_vecs == 0) return FLSGPU_OK;
+  const uint64_t blocks = (n_waves_for<UT, S>(a.n_vecs) + (BLOCK / WAVE) - 1) / (BLOCK / WAVE);
+  if (blocks > 0x7FFFFFFFull) return set_error(FLSGPU_EINVAL, "column too large");
+  hipLaunchKernelGGL((k_alprd_decompress_staged<T, S>), dim3(unsigned(blocks)), dim3(BLOCK), 0,
+                     stream, a, out);
+  return check_hip(hipGetLastError(), "k_alprd_decompress_staged launch");
 }
 
 template <typename T, bool NT>

@@ -25,6 +25,7 @@ static int resolve_variant(int variant, int dtype_bits, VariantShape *vs) {
     case FLSGPU_VARIANT_QUARTER_LANE_NT: *vs = {4, true, false}; return FLSGPU_OK;
     case FLSGPU_VARIANT_HALF_LANE_STAGED: *vs = {dtype_bits == 64 ? 4 : 2, true, true}; return FLSGPU_OK;
     case FLSGPU_VARIANT_QUARTER_LANE_STAGED: *vs = {4, true, true}; return FLSGPU_OK;
+    case FLSGPU_VARIANT_EIGHTH_LANE_STAGED: *vs = {8, true, true}; return FLSGPU_OK;
     default: return set_error(FLSGPU_EUNSUPPORTED, "unknown variant %d", variant);
   }
 }
@@ -52,8 +53,10 @@ static int launch_decompress(const FforArgs<UT> &a, UT *out, int variant, hipStr
     if constexpr (lanes<UT>::W == 32) {
       if (vs.S == 2) return FlsDecompressStaged<UT, 2>::launch(a, out, stream);
     }
+    if (vs.S == 8) return FlsDecompressStaged<UT, 8>::launch(a, out, stream);
     return FlsDecompressStaged<UT, 4>::launch(a, out, stream);
   }
+  if (vs.S == 8) vs.S = 4;  // unaligned output: the NT variant of the nearest split
   if (vs.S == 1) return vs.nt ? FlsDecompress<UT, 1, true>::launch(a, out, stream)
                               : FlsDecompress<UT, 1, false>::launch(a, out, stream);
   if (vs.S == 2) return vs.nt ? FlsDecompress<UT, 2, true>::launch(a, out, stream)

@@ -12,7 +12,7 @@
 // branchless unpackers rely on CUDA clamping `x >> 32` to 0, which AMD
 // hardware does not do: v_lshrrev_b32 masks the count to 5 bits). A value
 // straddling two words becomes one v_alignbit_b32 (or a 64-bit funnel).
-// A lane may be split over S threads (S = 1, 2, 4); each handles a
+// A lane may be split over S threads (S = 1, 2, 4, 8); each handles a
 // contiguous, compile-time slot range [J0, J0 + NJ) and loads only the words
 // that range touches.
 #pragma once
@@ -79,13 +79,24 @@ __device__ __forceinline__ void dispatch_part(Policy &pol, int part) {
   } else if constexpr (S == 2) {
     if (part == 0) pol.template run<BW, 0, NJ>();
     else pol.template run<BW, NJ, NJ>();
-  } else {
-    static_assert(S == 4, "S must be 1, 2 or 4");
+  } else if constexpr (S == 4) {
     switch (part) {
       case 0: pol.template run<BW, 0, NJ>(); break;
       case 1: pol.template run<BW, NJ, NJ>(); break;
       case 2: pol.template run<BW, 2 * NJ, NJ>(); break;
       default: pol.template run<BW, 3 * NJ, NJ>(); break;
+    }
+  } else {
+    static_assert(S == 8, "S must be 1, 2, 4 or 8");
+    switch (part) {
+      case 0: pol.template run<BW, 0, NJ>(); break;
+      case 1: pol.template run<BW, NJ, NJ>(); break;
+      case 2: pol.template run<BW, 2 * NJ, NJ>(); break;
+      case 3: pol.template run<BW, 3 * NJ, NJ>(); break;
+      case 4: pol.template run<BW, 4 * NJ, NJ>(); break;
+      case 5: pol.template run<BW, 5 * NJ, NJ>(); break;
+      case 6: pol.template run<BW, 6 * NJ, NJ>(); break;
+      default: pol.template run<BW, 7 * NJ, NJ>(); break;
     }
   }
 }

@@ -28,6 +28,7 @@ VARIANT_QUARTER_LANE = 5
 VARIANT_QUARTER_LANE_NT = 6
 VARIANT_HALF_LANE_STAGED = 7
 VARIANT_QUARTER_LANE_STAGED = 8
+VARIANT_EIGHTH_LANE_STAGED = 9
 VARIANTS = {
     "lane": VARIANT_LANE,
     "lane-nt": VARIANT_LANE_NT,
@@ -37,6 +38,7 @@ VARIANTS = {
     "quarter-lane-nt": VARIANT_QUARTER_LANE_NT,
     "half-lane-staged": VARIANT_HALF_LANE_STAGED,
     "quarter-lane-staged": VARIANT_QUARTER_LANE_STAGED,
+    "eighth-lane-staged": VARIANT_EIGHTH_LANE_STAGED,
 }
 
 PATCHERS = {

@@ -113,6 +113,19 @@ def tune_alp(values_log2: int = 24, reps: int = 10, device="cuda"):
                 if best is None or ms < best[1]:
                     best = (v, ms)
             out[name] = {"255": best[0]}
+    for width in (32, 64):  # ALP-RD: uniform values in [0, 1)
+        dt = np.float32 if width == 32 else np.float64
+        y = rng.uniform(0.0, 1.0, 1 << values_log2).astype(dt)
+        d = AL.DeviceAlprdColumn.from_host(AL.alprd_encode(y), device)
+        res = torch.empty(len(y), dtype=torch.float32 if width == 32 else torch.float64, device=device)
+        best = None
+        name = f"alprd_f{width}"
+        for vname, v in A.VARIANTS.items():
+            ms = _time(lambda: AL.alprd_decompress(d, res, variant=v), reps)
+            TIMES.setdefault(f"{name}/255", {})[vname] = round(ms, 4)
+            if best is None or ms < best[1]:
+                best = (v, ms)
+        out[name] = {"255": best[0]}
     return out
 
 

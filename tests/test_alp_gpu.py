@@ -113,7 +113,8 @@ def test_many_exceptions_per_lane(width):
     exp = oracle_decode(col)
     d = AL.DeviceAlpColumn.from_host(col)
     g = AL.DeviceAlpColumn.from_host(AL.galp_convert(col))
-    for variant in (0, A.VARIANT_HALF_LANE_STAGED, A.VARIANT_QUARTER_LANE_STAGED):
+    for variant in (0, A.VARIANT_HALF_LANE_STAGED, A.VARIANT_QUARTER_LANE_STAGED,
+                    A.VARIANT_EIGHTH_LANE_STAGED):
         got = AL.decompress(d, variant=variant).cpu().numpy()
         assert np.array_equal(bits(got), bits(exp)), f"variant {variant}"
         got = AL.decompress(g, variant=variant).cpu().numpy()
@@ -129,7 +130,8 @@ def test_staged_variants_misaligned_output(width):
     exp = oracle_decode(col)
     d = AL.DeviceAlpColumn.from_host(col)
     dt = torch.float32 if width == 32 else torch.float64
-    for variant in (A.VARIANT_HALF_LANE_STAGED, A.VARIANT_QUARTER_LANE_STAGED):
+    for variant in (A.VARIANT_HALF_LANE_STAGED, A.VARIANT_QUARTER_LANE_STAGED,
+                    A.VARIANT_EIGHTH_LANE_STAGED):
         buf = torch.full((col.n_vecs * 1024 + 1,), -1.0, dtype=dt, device="cuda")
         out = buf[1:]
         assert out.data_ptr() % 16 != 0

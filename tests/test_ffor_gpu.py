@@ -115,7 +115,8 @@ def test_staged_variants_misaligned_output(width):
     col = gpu_column(packed, bws, offs, bases, width)
     exp = oracle_decode(packed, bws, offs, bases, width)
     dt = torch.uint32 if width == 32 else torch.uint64
-    for variant in (A.VARIANT_HALF_LANE_STAGED, A.VARIANT_QUARTER_LANE_STAGED):
+    for variant in (A.VARIANT_HALF_LANE_STAGED, A.VARIANT_QUARTER_LANE_STAGED,
+                    A.VARIANT_EIGHTH_LANE_STAGED):
         buf = torch.zeros(n_vecs * 1024 + 2, dtype=dt, device="cuda")
         out = buf[1:-1]
         assert out.data_ptr() % 16 != 0
