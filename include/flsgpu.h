@@ -353,6 +353,19 @@ int flsgpu_file_load(const char *path, int encoding, void *column, void **d_bloc
 int flsgpu_file_read_host(const char *path, int encoding, void *column, void **h_block);
 void flsgpu_file_free_host(void *h_block);
 
+/* host::*Column::copy_to_device and free_column (src/flsgpu/structs.cuh:
+ * 103-229, 334-400) for any encoding: copies a column whose descriptor holds
+ * HOST pointers into one new device block (*d_block) and fills
+ * `device_column` (the descriptor type of `encoding`) with device pointers
+ * into it. The block keeps the reference's over-read padding after the packed
+ * words (sizeof(T) * N_LANES * 4 elements) and the exception arrays (4
+ * elements), zero-filled, for kernels written against the reference layout;
+ * this library's kernels do not need it. Returns after the copies completed
+ * on `stream`. flsgpu_column_free releases the block. */
+int flsgpu_column_upload(int encoding, const void *host_column, void *device_column,
+                         void **d_block, flsgpu_stream_t stream);
+int flsgpu_column_free(void *d_block);
+
 #ifdef __cplusplus
 }
 #endif

@@ -102,10 +102,19 @@ struct Writer {
   Header h{};
   std::vector<Pending> parts;
   uint64_t cursor = 0;
+  // Upload layout: the reference's over-read padding after the packed words
+  // (sizeof(T) * N_LANES * 4 elements, src/flsgpu/structs.cuh:104-109) and
+  // the exception arrays (MAX_UNPACK_N_VECS elements, :160,173-177). The
+  // kernels here never read it; callers' own kernels written for the
+  // reference layout may. Files carry no padding.
+  bool pad = false;
   void add(uint32_t id, uint32_t elem_bytes, uint64_t count, const void *data) {
+    uint64_t extra = 0;
+    if (pad && (id == S_PACKED || id == S_RIGHT_PACKED)) extra = 512;  // (W/8) * (1024/W) * 4
+    if (pad && (id == S_EXCEPTIONS || id == S_POSITIONS)) extra = 4;
     Pending p{{id, elem_bytes, count, cursor}, data};
     parts.push_back(p);
-    cursor = align_up(cursor + uint64_t(elem_bytes) * count);
+    cursor = align_up(cursor + uint64_t(elem_bytes) * (count + extra));
   }
 };
 
@@ -476,6 +485,45 @@ int flsgpu_file_write(const char *path, int encoding, const void *host_column) {
     return set_error(FLSGPU_EINVAL, "write failed: %s", path);
   if (fflush(f.get()) != 0) return set_error(FLSGPU_EINVAL, "write failed: %s", path);
   return FLSGPU_OK;
+}
+
+int flsgpu_column_upload(int encoding, const void *host_column, void *device_column,
+                         void **d_block, flsgpu_stream_t stream) {
+  if (!device_column || !d_block) return set_error(FLSGPU_EINVAL, "null argument");
+  *d_block = nullptr;
+  Writer w;
+  w.pad = true;
+  int rc = plan_columns(encoding, host_column, &w);
+  if (rc) return rc;
+  const uint64_t total = w.h.payload_bytes;
+  void *dev = nullptr;
+  if ((rc = check_hip(hipMalloc(&dev, total ? total : 1), "hipMalloc"))) return rc;
+  char *base = static_cast<char *>(dev);
+  rc = check_hip(hipMemsetAsync(dev, 0, total ? total : 1, stream), "hipMemsetAsync");
+  for (size_t i = 0; !rc && i < w.parts.size(); ++i) {
+    const Pending &p = w.parts[i];
+    const uint64_t bytes = uint64_t(p.s.elem_bytes) * p.s.count;
+    if (bytes)
+      rc = check_hip(hipMemcpyAsync(base + p.s.offset, p.data, bytes, hipMemcpyHostToDevice, stream),
+                     "hipMemcpyAsync H2D");
+  }
+  if (!rc) rc = check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  if (rc) {
+    (void)hipFree(dev);
+    return rc;
+  }
+  Parsed p;
+  p.h = w.h;
+  kind_of(encoding, &p.k);
+  p.file_bytes = 0;
+  describe(p, base, device_column);
+  *d_block = dev;
+  return FLSGPU_OK;
+}
+
+int flsgpu_column_free(void *d_block) {
+  if (!d_block) return FLSGPU_OK;
+  return check_hip(hipFree(d_block), "hipFree");
 }
 
 int flsgpu_file_stat(const char *path, flsgpu_file_info *info) {

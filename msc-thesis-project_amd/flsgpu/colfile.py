@@ -1,6 +1,7 @@
 """Column files (.flsc) through the library (include/flsgpu.h,
 flsgpu_file_*): write a host column, read it back on the host, or load it
-into one device block and decode / query it there.
+into one device block and decode / query it there; and uploads of host
+columns into one device block (flsgpu_column_upload / flsgpu_column_free).
 
 Host columns are the encoder's outputs: AlpHostColumn (standard layout, or
 G-ALP when offsets_counts is set), AlprdHostColumn, and FforHostColumn below
@@ -43,6 +44,10 @@ def _lib():
         L.flsgpu_file_read_host.restype = C.c_int
         L.flsgpu_file_free_host.argtypes = [_P]
         L.flsgpu_file_free_host.restype = None
+        L.flsgpu_column_upload.argtypes = [C.c_int, _P, _P, C.POINTER(_P), _P]
+        L.flsgpu_column_upload.restype = C.c_int
+        L.flsgpu_column_free.argtypes = [_P]
+        L.flsgpu_column_free.restype = C.c_int
         L._file_sigs = True
     return L
 
@@ -194,17 +199,16 @@ def read_host(path: str):
         _lib().flsgpu_file_free_host(block)
 
 
-class LoadedColumn:
-    """A column file loaded into one device block (freed with the object)."""
+class BlockColumn:
+    """A column held in one device block owned by the library (a loaded
+    column file or an uploaded host column); freed with the object."""
 
-    def __init__(self, path: str, stream=None):
-        from .device import _stream_ptr
-        self.encoding = ENC[stat(path)["encoding"]]
+    def __init__(self, encoding: int, fill):
+        self.encoding = encoding
         self.name = ENC_NAME[self.encoding]
         self.struct = _struct_for(self.encoding)
         self._block = _P()
-        A.check(_lib().flsgpu_file_load(path.encode(), self.encoding, C.byref(self.struct),
-                                        C.byref(self._block), _stream_ptr(stream)))
+        A.check(fill(self.struct, self._block))
         self.width = int(self.name[-2:])
         bp = (self.struct.ffor.bp if self.name.startswith("alp_") else
               self.struct.bp if self.name.startswith("ffor_") else self.struct)
@@ -212,7 +216,7 @@ class LoadedColumn:
 
     def __del__(self):
         if getattr(self, "_block", None) and self._block.value:
-            A.lib().flsgpu_free(self._block)
+            _lib().flsgpu_column_free(self._block)
             self._block = _P()
 
     def out_dtype(self):
@@ -259,5 +263,24 @@ class LoadedColumn:
         return r
 
 
-def load(path: str, stream=None) -> LoadedColumn:
-    return LoadedColumn(path, stream)
+def load(path: str, stream=None) -> BlockColumn:
+    """flsgpu_file_load: the column file's payload copied into one device block."""
+    from .device import _stream_ptr
+    enc = ENC[stat(path)["encoding"]]
+    return BlockColumn(enc, lambda st, blk: _lib().flsgpu_file_load(
+        path.encode(), enc, C.byref(st), C.byref(blk), _stream_ptr(stream)))
+
+
+def upload(col, stream=None) -> BlockColumn:
+    """flsgpu_column_upload: a host column copied into one device block with
+    the reference's over-read padding (host::*Column::copy_to_device)."""
+    from .device import _stream_ptr
+    enc = encoding_of(col)
+    host, keep = _descriptor(col)
+
+    def fill(st, blk):
+        rc = _lib().flsgpu_column_upload(enc, C.byref(host), C.byref(st), C.byref(blk),
+                                         _stream_ptr(stream))
+        del keep[:]
+        return rc
+    return BlockColumn(enc, fill)

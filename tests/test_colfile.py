@@ -47,10 +47,13 @@ def columns(rng):
     for w in (32, 64):
         out.append(ffor_column(rng, w, 7, True))
         out.append(ffor_column(rng, w, 5, False))
-        col = AL.alp_encode(decimal_values(rng, w, 12 * 1024))
+        dt = np.float32 if w == 32 else np.float64
+        x = rng.integers(-10**4, 10**4, 12 * 1024).astype(dt)
+        x[::97] += dt(0.37)  # exceptions; integer-valued data always samples as ALP
+        col = AL.alp_encode(x)
+        assert col is not None
         out.append(col)
         out.append(AL.galp_convert(col))
-        dt = np.float32 if w == 32 else np.float64
         out.append(AL.alprd_encode(rng.random(6 * 1024).astype(dt)))
     return out
 

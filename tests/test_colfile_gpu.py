@@ -78,6 +78,33 @@ def test_load_larger_than_one_staging_chunk(tmp_path):
     os.remove(p)
 
 
+def test_upload_every_encoding_with_padding(tmp_path):
+    """flsgpu_column_upload (host::*Column::copy_to_device): decodes like the
+    array-by-array upload; the reference's over-read padding after the packed
+    words is there and zero."""
+    import ctypes as C
+    from flsgpu import _capi as A
+    rng = np.random.default_rng(21)
+    for col in TC.columns(rng):
+        dev = CF.upload(col)
+        got = dev.decompress().cpu().numpy()
+        if isinstance(col, CF.FforHostColumn):
+            assert np.array_equal(got, expected(col))
+            words = len(col.packed)
+            pad = np.ones(512, dtype=col.packed.dtype)
+            bp = dev.struct.bp if col.bases is not None else dev.struct
+            A.check(A.lib().flsgpu_download(pad.ctypes.data_as(C.c_void_p),
+                                            C.c_void_p(bp.packed_array + words * col.packed.itemsize),
+                                            pad.nbytes, None))
+            assert not pad.any()
+        else:
+            w = 32 if got.dtype == np.float32 else 64
+            ref = (AL.alprd_decompress(AL.DeviceAlprdColumn.from_host(col))
+                   if isinstance(col, AL.AlprdHostColumn) else
+                   AL.decompress(AL.DeviceAlpColumn.from_host(col))).cpu().numpy()
+            assert np.array_equal(got.view(UT[w]), ref.view(UT[w]))
+
+
 @pytest.mark.parametrize("compressor", ["ALP", "GALP", "ALPRD"])
 @pytest.mark.parametrize("dtype", ["f32", "f64"])
 def test_end_to_end_driver(tmp_path, compressor, dtype, capsys):
