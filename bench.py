@@ -64,7 +64,8 @@ def main():
     stream = torch.cuda.current_stream()
     nvals = 1 << args.values_log2
     n_vecs = nvals // 1024
-    print(f"[bench] building sweep columns: 2^{args.values_log2} values per width", flush=True)
+    print(f"[bench] building sweep columns: 2^{args.values_log2} values per width",
+          file=sys.stderr, flush=True)
 
     cols = []  # (width, bw, DeviceFFORColumn)
     for width in (32, 64):
@@ -75,7 +76,8 @@ def main():
     out32 = torch.empty(nvals, dtype=torch.uint32, device=dev)
     out64 = torch.empty(nvals, dtype=torch.uint64, device=dev)
     torch.cuda.synchronize()
-    print(f"[bench] resident: {torch.cuda.memory_allocated() / 1e9:.1f} GB", flush=True)
+    print(f"[bench] resident: {torch.cuda.memory_allocated() / 1e9:.1f} GB", file=sys.stderr,
+          flush=True)
 
     def algo_bytes(width, bw):
         nl = 1024 // width
