@@ -20,6 +20,12 @@
 
 namespace flsgpu_hip {
 
+// One thread per lane. Exceptions are patched in registers: bit j of `emask`
+// marks slot j of this lane as an exception whose left part is the next
+// 16 bits of `lq` (the lane's first RD_REG exceptions in slot order), so
+// the value is stored once, already patched.
+constexpr int RD_REG = 4;
+
 template <typename T, bool NT>
 struct RdPolicy {
   using UT = typename uint_of<T>::type;
@@ -30,12 +36,16 @@ struct RdPolicy {
   const UT *dshift;  // LDS, 8 entries
   uint64_t lstream[Q];
   unsigned lbw;
+  uint64_t emask;
+  uint64_t lq;
   template <int BW, int J0, int NJ>
   __device__ __forceinline__ void run() {
     const uint64_t lmask = (uint64_t(1) << lbw) - 1;
     unpack_slots<UT, BW, J0, NJ>(rin, [&](int j, UT r) {
       const unsigned idx = unsigned(lstream[j % Q] >> ((j / Q) * lbw)) & unsigned(lmask);
-      const UT v = dshift[idx] | r;
+      const bool ex = (emask >> j) & 1u;
+      const UT v = ex ? UT(UT(UT(lq & 0xFFFFu) << BW) | r) : UT(dshift[idx] | r);
+      lq = ex ? (lq >> 16) : lq;
       store<T, NT>(out + j * NL, __builtin_bit_cast(T, v));
     });
   }
@@ -67,39 +77,107 @@ __device__ __forceinline__ void alprd_fill_dictionary(const AlprdArgs<T> &a,
     dshift[threadIdx.x] = threadIdx.x < a.dict_size ? UT(UT(a.dict[threadIdx.x]) << a.right_bw) : UT(0);
 }
 
+// Exception entries per vector that the one-thread-per-lane decoder stages in
+// LDS: the vector's N_LANES threads load the first RD_LIST entries of its
+// list together (64 / N_LANES each, coalesced) as (left << 16) | position.
+// Every thread then reads the staged list for its lane's entries: the first
+// RD_REG (in slot order) are patched in registers during the decode; any
+// further ones, and entries past RD_LIST, are stored after the decode (the
+// same thread's later store to the same address wins). Reading the whole
+// list from global memory per thread and patching after the decode made the
+// time grow with the exception count (2^26 doubles: 201 us at 4 exceptions
+// per vector, 289 us at 18).
+constexpr unsigned RD_LIST = 64;
+
+// `list`: this wave's LDS, VPW * RD_LIST entries. Every lane of the wave
+// calls this (the LDS hand-over is wave-wide); lanes past the column end
+// decode nothing.
 template <typename T, bool NT>
 __device__ __forceinline__ void alprd_decompress_unit(const AlprdArgs<T> &a, T *__restrict out,
                                                       uint64_t wave,
-                                                      const typename uint_of<T>::type *dshift) {
+                                                      const typename uint_of<T>::type *dshift,
+                                                      uint32_t *list) {
   using UT = typename uint_of<T>::type;
   using L = lanes<UT>;
   constexpr int Q = 64 / L::N;
+  constexpr int PFN = int(RD_LIST) / L::N;
   const UnitMap<UT, 1> u = map_unit<UT, 1>(wave);
-  if (u.vec >= a.n_vecs) return;
+  const int vv = int(threadIdx.x % WAVE) / L::N;
+  bool live = u.vec < a.n_vecs;
+  uint64_t eo = 0;
+  unsigned count = 0;
+  uint32_t ent[PFN];
+#pragma unroll
+  for (int i = 0; i < PFN; ++i) ent[i] = 0;
   T *vout = out + u.vec * VALUES_PER_VECTOR;
   const UT *rin = a.right + u.vec * uint64_t(a.right_bw) * L::N + u.lane;
   RdPolicy<T, NT> pol;
-  pol.rin = rin;
-  pol.out = vout + u.lane;
-  pol.dshift = dshift;
-  pol.lbw = a.left_bw;
-  const uint16_t *lin = a.left + u.vec * uint64_t(a.left_bw) * 64;
+  if (live) {
+    eo = a.exceptions_offsets[u.vec];
+    count = a.counts[u.vec];
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    uint64_t s = 0;
-    for (unsigned r = 0; r < a.left_bw; ++r)
-      s |= uint64_t(lin[u.lane + L::N * q + 64 * r]) << (16 * r);
-    pol.lstream[q] = s;
+    for (int i = 0; i < PFN; ++i) {
+      const unsigned k = unsigned(u.lane + L::N * i);
+      if (k < count)
+        ent[i] = (a.positions[eo + k] & 1023u) | (uint32_t(a.exceptions[eo + k]) << 16);
+    }
+    const uint16_t *lin = a.left + u.vec * uint64_t(a.left_bw) * 64;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      uint64_t s = 0;
+      for (unsigned r = 0; r < a.left_bw; ++r)
+        s |= uint64_t(lin[u.lane + L::N * q + 64 * r]) << (16 * r);
+      pol.lstream[q] = s;
+    }
   }
-  if (!dispatch_right_bw<UT>(pol, int(a.right_bw))) return;  // rejected on the host
-  // Every thread walks the vector's exception list and keeps its lane's; the
-  // list is read RD_PF entries at a time with all loads in flight (one load
-  // at a time made the decode time grow with the exception count: 4 -> 18
-  // exceptions per vector took a 2^26-double column from 206 to 320 us).
+#pragma unroll
+  for (int i = 0; i < PFN; ++i) list[vv * RD_LIST + u.lane + L::N * i] = ent[i];
+  wave_lds_sync();
+  const uint32_t *vlist = list + vv * RD_LIST;
+  const unsigned n_list = count < RD_LIST ? count : RD_LIST;
+  // this lane's exception slots among the staged entries
+  uint64_t own = 0;
+  for (unsigned k = 0; k < n_list; ++k) {
+    const unsigned pos = vlist[k] & 1023u;
+    if (int(pos % L::N) == u.lane) own |= uint64_t(1) << (pos / L::N);
+  }
+  // the first RD_REG of them go to registers, in slot order
+  uint64_t emask = own;
+  for (int extra = __builtin_popcountll(own) - RD_REG; extra > 0; --extra)
+    emask &= ~(uint64_t(1) << (63 - __builtin_clzll(emask)));
+  uint64_t lq = 0;
+  if (emask) {
+    for (unsigned k = 0; k < n_list; ++k) {
+      const uint32_t e = vlist[k];
+      const unsigned pos = e & 1023u;
+      const unsigned slot = pos / L::N;
+      if (int(pos % L::N) != u.lane || !((emask >> slot) & 1u)) continue;
+      const int rank = __builtin_popcountll(emask & ((uint64_t(1) << slot) - 1));
+      lq = (lq & ~(uint64_t(0xFFFFu) << (16 * rank))) | (uint64_t(e >> 16) << (16 * rank));
+    }
+  }
+  if (live) {
+    pol.rin = rin;
+    pol.out = vout + u.lane;
+    pol.dshift = dshift;
+    pol.lbw = a.left_bw;
+    pol.emask = emask;
+    pol.lq = lq;
+    live = dispatch_right_bw<UT>(pol, int(a.right_bw));  // false only for widths rejected on the host
+  }
+  if (!live) return;
+  // entries the registers did not take: stored over the decoded values
+  if (own != emask) {
+    for (unsigned k = 0; k < n_list; ++k) {
+      const uint32_t e = vlist[k];
+      const unsigned pos = e & 1023u;
+      if (int(pos % L::N) != u.lane || ((emask >> (pos / L::N)) & 1u)) continue;
+      const UT right = extract_runtime<UT>(rin, int(a.right_bw), int(pos / L::N));
+      store<T, NT>(vout + pos, __builtin_bit_cast(T, UT(UT(UT(e >> 16) << a.right_bw) | right)));
+    }
+  }
   constexpr unsigned RD_PF = 8;
-  const uint64_t eo = a.exceptions_offsets[u.vec];
-  const unsigned count = a.counts[u.vec];
-  for (unsigned k0 = 0; k0 < count; k0 += RD_PF) {
+  for (unsigned k0 = RD_LIST; k0 < count; k0 += RD_PF) {
     unsigned pos[RD_PF];
     uint16_t left[RD_PF];
 #pragma unroll
@@ -122,10 +200,13 @@ __device__ __forceinline__ void alprd_decompress_unit(const AlprdArgs<T> &a, T *
 
 template <typename T, bool NT>
 __global__ __launch_bounds__(BLOCK) void k_alprd_decompress(AlprdArgs<T> a, T *__restrict out) {
-  __shared__ typename uint_of<T>::type dshift[8];
+  using UT = typename uint_of<T>::type;
+  __shared__ UT dshift[8];
+  __shared__ uint32_t s_list[BLOCK / WAVE][lanes<UT>::VPW * RD_LIST];
   alprd_fill_dictionary<T>(a, dshift);
   __syncthreads();
-  alprd_decompress_unit<T, NT>(a, out, (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE, dshift);
+  alprd_decompress_unit<T, NT>(a, out, (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE, dshift,
+                               s_list[threadIdx.x / WAVE]);
 }
 
 template <typename T>

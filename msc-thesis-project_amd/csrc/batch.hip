@@ -40,38 +40,42 @@ struct BatchColumn {
   } u;
 };
 
-// Per-encoding task bodies are separate (non-inlined) functions: the
-// compiler then optimises each decoder on its own instead of one huge CFG,
-// which keeps build time linear in the number of encodings. A call per task
-// (16 wave-units) costs nothing measurable.
+// Per-encoding task bodies are inlined into the kernel. As separate
+// (non-inlined) functions they follow the call convention, whose
+// callee-saved VGPRs sit in blocks of 8 across the whole register file: a
+// 64-bit body that needs ~130 caller-saved registers then touches v0..v247
+// and the kernel ran one wave per SIMD. Inlined, with a three-wave target,
+// the kernel takes 168 VGPRs (a few bytes of spill on the widest paths).
 template <typename UT, bool NT>
-__device__ __noinline__ void batch_ffor(const BatchColumn *c, uint64_t w0, int wib) {
+__device__ __forceinline__ void batch_ffor(const BatchColumn *c, uint64_t w0, int wib) {
   const FforArgs<UT> &a = *reinterpret_cast<const FforArgs<UT> *>(&c->u);
   for (int k = 0; k < TASK_WAVES / 4; ++k)
     ffor_decompress_unit<UT, 1, NT>(a, static_cast<UT *>(c->out), w0 + wib + 4 * k);
 }
 
 template <typename T, int LAYOUT, bool NT>
-__device__ __noinline__ void batch_alp(const BatchColumn *c, uint64_t w0, int wib) {
+__device__ __forceinline__ void batch_alp(const BatchColumn *c, uint64_t w0, int wib) {
   const AlpArgs<T> &a = *reinterpret_cast<const AlpArgs<T> *>(&c->u);
   for (int k = 0; k < TASK_WAVES / 4; ++k)
     alp_decompress_unit_scan<T, NT, LAYOUT>(a, static_cast<T *>(c->out), w0 + wib + 4 * k);
 }
 
 template <typename T, bool NT>
-__device__ __noinline__ void batch_alprd(const BatchColumn *c, uint64_t w0, int wib,
-                                         const typename uint_of<T>::type *dict) {
+__device__ __forceinline__ void batch_alprd(const BatchColumn *c, uint64_t w0, int wib,
+                                         const typename uint_of<T>::type *dict, uint32_t *list) {
   const AlprdArgs<T> &a = *reinterpret_cast<const AlprdArgs<T> *>(&c->u);
   for (int k = 0; k < TASK_WAVES / 4; ++k)
-    alprd_decompress_unit<T, NT>(a, static_cast<T *>(c->out), w0 + wib + 4 * k, dict);
+    alprd_decompress_unit<T, NT>(a, static_cast<T *>(c->out), w0 + wib + 4 * k, dict, list);
 }
 
 template <bool NT>
-__global__ __launch_bounds__(BLOCK) void k_batch(const BatchColumn *__restrict cols,
-                                                 const uint32_t *__restrict task_col,
-                                                 uint64_t n_tasks, unsigned long long *queue) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_batch(
+    const BatchColumn *__restrict cols, const uint32_t *__restrict task_col, uint64_t n_tasks,
+    unsigned long long *queue) {
   __shared__ unsigned long long s_task;
   __shared__ uint64_t dict_lds[8];
+  static_assert(lanes<uint32_t>::VPW <= lanes<uint64_t>::VPW, "rd_list sized for the widest VPW");
+  __shared__ uint32_t rd_list[BLOCK / WAVE][lanes<uint64_t>::VPW * RD_LIST];
   const int wib = int(threadIdx.x / WAVE);
   for (;;) {
     __syncthreads();  // every wave is done with the previous task (and its LDS)
@@ -94,13 +98,13 @@ __global__ __launch_bounds__(BLOCK) void k_batch(const BatchColumn *__restrict c
         uint32_t *d = reinterpret_cast<uint32_t *>(dict_lds);
         alprd_fill_dictionary<float>(c->u.r32, d);
         __syncthreads();
-        batch_alprd<float, NT>(c, w0, wib, d);
+        batch_alprd<float, NT>(c, w0, wib, d, rd_list[wib]);
         break;
       }
       case FLSGPU_ENC_ALPRD_F64:
         alprd_fill_dictionary<double>(c->u.r64, dict_lds);
         __syncthreads();
-        batch_alprd<double, NT>(c, w0, wib, dict_lds);
+        batch_alprd<double, NT>(c, w0, wib, dict_lds, rd_list[wib]);
         break;
       default:
         break;

@@ -157,16 +157,25 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_TABLE)
     ap.add_argument("--values-log2", type=int, default=28)
+    ap.add_argument("--only", choices=["all", "ffor", "alp"], default="all",
+                    help="re-tune these encodings; the others keep their entries in the "
+                         "shipped table")
     args = ap.parse_args()
     A.check(A.lib().flsgpu_init(0))
-    table = tune_ffor(args.values_log2)
-    # ALP columns: the C3 size (2^26 doubles / floats); encoding 2^28 values
-    # on the host would dominate the run
-    table.update(tune_alp(min(args.values_log2, 26)))
-    # the column default is the variant that wins most widths
-    for enc in ("ffor_u32", "ffor_u64"):
-        vals = list(table[enc].values())
-        table[enc]["255"] = max(set(vals), key=vals.count)
+    table = {}
+    if args.only != "all":
+        with open(DEFAULT_TABLE) as f:
+            table = json.load(f)
+    if args.only in ("all", "ffor"):
+        table.update(tune_ffor(args.values_log2))
+        # the column default is the variant that wins most widths
+        for enc in ("ffor_u32", "ffor_u64"):
+            vals = [v for k, v in table[enc].items() if k != "255"]
+            table[enc]["255"] = max(set(vals), key=vals.count)
+    if args.only in ("all", "alp"):
+        # ALP columns: the C3 size (2^26 doubles / floats); encoding 2^28
+        # values on the host would dominate the run
+        table.update(tune_alp(min(args.values_log2, 26)))
     with open(args.out, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
     with open(os.path.splitext(args.out)[0] + "_times.json", "w") as f:
