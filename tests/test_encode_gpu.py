@@ -43,7 +43,7 @@ def t2n(t, n=None):
 def columns(dtype, seed):
     rng = np.random.default_rng(seed)
     nv = 24
-    x = SEL.generate(nv, rng, 1024).astype(dtype)
+    x = SEL.generate(nv, rng, 1024).astype(dtype).reshape(-1)
     v = x.reshape(nv, 1024)
     v[0] = np.nan                                    # every value an exception
     v[1, :1023] = np.inf                             # only slot 1023 encodes
@@ -72,11 +72,17 @@ def params(x, dtype, rng, mode):
 @pytest.mark.parametrize("mode", ["search", "random"])
 @pytest.mark.parametrize("galp", [False, True])
 def test_alp_device_encode_matches_host(dtype, mode, galp):
+    if galp and mode == "random" and dtype == np.float64:
+        pytest.skip("random parameters leave whole f64 lanes (64 slots) as exceptions, "
+                    "which the 16-bit G-ALP lane header cannot express")
     rng = np.random.default_rng(3)
     x = columns(dtype, 11)
-    if galp:  # vectors 0 and 1 overflow the 16-bit G-ALP lane header
-        x.reshape(-1, 1024)[0] = np.round(x.reshape(-1, 1024)[6], 2)
-        x.reshape(-1, 1024)[1] = np.round(x.reshape(-1, 1024)[7], 1)
+    if galp:  # vectors with a lane of 64 exceptions cannot be expressed in G-ALP
+        v = x.reshape(-1, 1024)
+        v[0] = np.round(v[6], 2)
+        v[1] = np.round(v[7], 1)
+        if dtype == np.float64:
+            v[3] = np.round(v[8], 2)
     e, f = params(x, dtype, rng, mode)
     host = AL.alp_encode_with_params(x, e, f)
     if galp:
