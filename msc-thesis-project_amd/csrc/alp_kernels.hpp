@@ -8,11 +8,13 @@
 // contraction (built with -ffp-contract=off), no division by 10^e.
 //
 // Exception patching (src/flsgpu/alp.cuh:48-461, CPU decoder.hpp:137-145):
-// every exception is written by the thread that owns its (lane, slot), after
-// that thread stored the decoded value at the same address, so the final value
-// is the exception by single-thread program order. With the G-ALP layout a
-// thread reads only its own lane's exceptions through the per-lane header;
-// with the standard layout it scans the vector's list and keeps its lane's.
+// every exception is written by the thread that owns its (lane, slot). With
+// the G-ALP layout that thread reads its lane's header and loads its first few
+// exceptions before decoding, storing them in place of the decoded values;
+// further ones, and those of the standard layout (whose list the wave indexes
+// in LDS), it writes after it stored the decoded value at the same address, so
+// the final value is the exception by single-thread program order. The
+// LDS-staged kernels patch the staged values.
 #pragma once
 
 #include "alp_impl.hpp"
@@ -47,6 +49,45 @@ struct AlpDecompressPolicy {
   __device__ __forceinline__ void run_generic(int bw, int j0, int nj) {
     unpack_runtime<UT>(in, bw, j0, nj, [&](int j, UT v) {
       store<T, NT>(out + j * lanes<UT>::N, alp_value<T>(v, base, fact, frac));
+    });
+  }
+};
+
+// Decode with exceptions patched in registers (G-ALP layout): bit j of
+// `emask` marks slot j as an exception whose value is the head of the queue q
+// (filled in slot order), so each value is stored once. RK entries per
+// thread: a few exceptions per lane are typical (2 decimals, 1 % exceptions:
+// 0.6 per f64 lane; f32 columns of 1 decimal: 4.5).
+template <typename T>
+constexpr int alp_reg_k() { return sizeof(T) == 8 ? 4 : 8; }
+
+template <typename T, bool NT>
+struct AlpRegPatchPolicy {
+  using UT = typename uint_of<T>::type;
+  static constexpr int RK = alp_reg_k<T>();
+  const UT *in;
+  T *out;
+  UT base;
+  UT fact;
+  T frac;
+  uint64_t emask;
+  T q[RK];
+  __device__ __forceinline__ T take(int j, T y) {
+    const bool ex = (emask >> j) & 1u;
+    const T r = ex ? q[0] : y;
+#pragma unroll
+    for (int i = 0; i + 1 < RK; ++i) q[i] = ex ? q[i + 1] : q[i];
+    return r;
+  }
+  template <int BW, int J0, int NJ>
+  __device__ __forceinline__ void run() {
+    unpack_slots<UT, BW, J0, NJ>(in, [&](int j, UT v) {
+      store<T, NT>(out + j * lanes<UT>::N, take(j, alp_value<T>(v, base, fact, frac)));
+    });
+  }
+  __device__ __forceinline__ void run_generic(int bw, int j0, int nj) {
+    unpack_runtime<UT>(in, bw, j0, nj, [&](int j, UT v) {
+      store<T, NT>(out + j * lanes<UT>::N, take(j, alp_value<T>(v, base, fact, frac)));
     });
   }
 };
@@ -98,9 +139,10 @@ __device__ __forceinline__ void lds_or(UT *p, UT v) {
     atomicOr(reinterpret_cast<unsigned *>(p), static_cast<unsigned>(v));
 }
 
+// `skip`: bit k set for list entries k < 64 the caller has applied already.
 template <typename T, class F>
 __device__ __forceinline__ void galp_own_exceptions(const AlpArgs<T> &a, uint64_t vec, int lane,
-                                                    int j0, int nj, F &&f) {
+                                                    int j0, int nj, F &&f, uint64_t skip = 0) {
   constexpr int NL = lanes<typename uint_of<T>::type>::N;
   const uint64_t eo = a.exceptions_offsets[vec];
   const unsigned oc = a.offsets_counts[vec * NL + lane];
@@ -118,7 +160,7 @@ __device__ __forceinline__ void galp_own_exceptions(const AlpArgs<T> &a, uint64_
     }
 #pragma unroll
     for (int i = 0; i < ALP_PF; ++i) {
-      if (k0 + i < count) {
+      if (k0 + i < count && !(k0 + i < 64 && ((skip >> (k0 + i)) & 1u))) {
         const int slot = int(pos[i]) / NL;
         if (slot >= j0 && slot < j0 + nj) f(pos[i], val[i]);
       }
@@ -158,16 +200,32 @@ struct StdIndex {
     __builtin_amdgcn_wave_barrier();
   }
 
+  // Slot mask of (vector vv of the group, lane) in slots [j0, j0 + nj).
+  __device__ __forceinline__ UT slots(int vv, int lane, int j0, int nj) const {
+    constexpr int W = lanes<UT>::W;
+    UT m = lmask[vv * NL + lane];
+    if (nj < W) m &= UT(((UT(1) << nj) - 1) << j0);
+    return m;
+  }
+  // Index in vector vv's list of the exception at position p.
+  __device__ __forceinline__ unsigned rank(int vv, unsigned p) const {
+    const uint64_t *b = bits + vv * 16;
+    unsigned r = unsigned(__builtin_popcountll(b[p >> 6] & ((1ull << (p & 63)) - 1)));
+    for (unsigned w = 0; w < (p >> 6); ++w) r += unsigned(__builtin_popcountll(b[w]));
+    return r;
+  }
   // Exceptions of (vector vv of the group, lane) in slots [j0, j0 + nj).
   template <class F>
   __device__ __forceinline__ void own(const AlpArgs<T> &a, uint64_t vec, int vv, int lane, int j0,
                                       int nj, F &&f) const {
-    constexpr int W = lanes<UT>::W;
-    UT m = lmask[vv * NL + lane];
-    if (nj < W) m &= UT(((UT(1) << nj) - 1) << j0);
+    own_slots(a, vec, vv, lane, slots(vv, lane, j0, nj), f);
+  }
+  // The exceptions at the slots set in m of (vector vv of the group, lane).
+  template <class F>
+  __device__ __forceinline__ void own_slots(const AlpArgs<T> &a, uint64_t vec, int vv, int lane,
+                                            UT m, F &&f) const {
     if (!m) return;
     const uint64_t eo = a.exceptions_offsets[vec];
-    const uint64_t *b = bits + vv * 16;
     while (m) {
       unsigned pos[ALP_PF], rank[ALP_PF];
       int n = 0;
@@ -177,10 +235,8 @@ struct StdIndex {
           const int j = int(__builtin_ctzll(uint64_t(m)));
           m &= UT(m - 1);
           const unsigned p = unsigned(lane + NL * j);
-          unsigned r = unsigned(__builtin_popcountll(b[p >> 6] & ((1ull << (p & 63)) - 1)));
-          for (unsigned w = 0; w < (p >> 6); ++w) r += unsigned(__builtin_popcountll(b[w]));
           pos[i] = p;
-          rank[i] = r;
+          rank[i] = this->rank(vv, p);
           n = i + 1;
         }
       }
@@ -249,6 +305,7 @@ __device__ __forceinline__ void alp_decompress_unit(const AlpArgs<T> &a, T *__re
   using L = lanes<UT>;
   const UnitMap<UT, S> u = map_unit<UT, S>(wave);
   const int j0 = u.part * (L::W / S);
+  constexpr int NJ = L::W / S;
   if constexpr (PATCH && LAYOUT == ALP_STD) {
     __shared__ uint64_t s_bits[BLOCK / WAVE][L::VPW * 16];
     __shared__ UT s_lmask[BLOCK / WAVE][L::VPW * L::N];
@@ -257,15 +314,68 @@ __device__ __forceinline__ void alp_decompress_unit(const AlpArgs<T> &a, T *__re
     idx.build(a, wave / S);
     if (u.vec >= a.ffor.n_vecs) return;
     T *vout = out + u.vec * VALUES_PER_VECTOR;
+    // Patched after the decode: register patching (as for G-ALP below) was
+    // slower here, the LDS index build before the decode dominates
+    // (2^26 doubles, 10 exceptions per vector: 134 -> 137 us).
     AlpDecompressPolicy<T, NT> pol{a.ffor.packed + a.ffor.vector_offsets[u.vec] + u.lane,
                                    vout + u.lane, a.ffor.bases[u.vec],
                                    alp_factor<T>(a.factor_indices[u.vec]),
                                    alp_fraction<T>(a.fraction_indices[u.vec])};
     if (!dispatch_bw<UT, S>(pol, a.ffor.bit_widths[u.vec], u.part)) {
-      for (int j = j0; j < j0 + L::W / S; ++j) vout[u.lane + j * L::N] = T(0);
+      for (int j = j0; j < j0 + NJ; ++j) vout[u.lane + j * L::N] = T(0);
     }
-    idx.own(a, u.vec, int((threadIdx.x % WAVE) / L::N), u.lane, j0, L::W / S,
+    idx.own(a, u.vec, int((threadIdx.x % WAVE) / L::N), u.lane, j0, NJ,
             [&](unsigned pos, T e) { store<T, NT>(vout + pos, e); });
+  } else if constexpr (PATCH) {
+    if (u.vec >= a.ffor.n_vecs) return;
+    T *vout = out + u.vec * VALUES_PER_VECTOR;
+    AlpRegPatchPolicy<T, NT> pol{a.ffor.packed + a.ffor.vector_offsets[u.vec] + u.lane,
+                                 vout + u.lane, a.ffor.bases[u.vec],
+                                 alp_factor<T>(a.factor_indices[u.vec]),
+                                 alp_fraction<T>(a.fraction_indices[u.vec]), 0, {}};
+    // the first RK entries of the lane's list: those in this thread's slot
+    // range go to the queue by slot rank (a repeated slot keeps the later
+    // entry); every other entry is stored after the decode
+    constexpr int RK = AlpRegPatchPolicy<T, NT>::RK;
+    const unsigned oc = a.offsets_counts[u.vec * L::N + u.lane];
+    const unsigned count = oc >> 10;
+    const uint64_t first = count ? a.exceptions_offsets[u.vec] + (oc & 0x3FFu) : 0;
+    unsigned slot[RK];
+    T val[RK];
+#pragma unroll
+    for (int k = 0; k < RK; ++k) {
+      slot[k] = 0xFFFFu;  // not queued
+      if (unsigned(k) < count) {
+        const unsigned pos = a.positions[first + k];
+        val[k] = a.exceptions[first + k];
+        const unsigned sl = pos / unsigned(L::N);
+        if (int(pos % unsigned(L::N)) == u.lane && sl >= unsigned(j0) && sl < unsigned(j0 + NJ))
+          slot[k] = sl;
+      }
+    }
+    uint64_t queued = 0;
+#pragma unroll
+    for (int k = 0; k < RK; ++k) {
+      pol.q[k] = T(0);
+      if (slot[k] != 0xFFFFu) {
+        pol.emask |= uint64_t(1) << slot[k];
+        queued |= uint64_t(1) << k;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < RK; ++k) {
+      if (slot[k] != 0xFFFFu) {
+        const int r = __builtin_popcountll(pol.emask & ((uint64_t(1) << slot[k]) - 1));
+#pragma unroll
+        for (int i = 0; i < RK; ++i) pol.q[i] = r == i ? val[k] : pol.q[i];
+      }
+    }
+    if (!dispatch_bw<UT, S>(pol, a.ffor.bit_widths[u.vec], u.part)) {
+      for (int j = j0; j < j0 + NJ; ++j) vout[u.lane + j * L::N] = T(0);
+      queued = 0;  // nothing was patched
+    }
+    galp_own_exceptions<T>(a, u.vec, u.lane, j0, NJ,
+                           [&](unsigned pos, T e) { store<T, NT>(vout + pos, e); }, queued);
   } else {
     if (u.vec >= a.ffor.n_vecs) return;
     T *vout = out + u.vec * VALUES_PER_VECTOR;
@@ -274,11 +384,7 @@ __device__ __forceinline__ void alp_decompress_unit(const AlpArgs<T> &a, T *__re
                                    alp_factor<T>(a.factor_indices[u.vec]),
                                    alp_fraction<T>(a.fraction_indices[u.vec])};
     if (!dispatch_bw<UT, S>(pol, a.ffor.bit_widths[u.vec], u.part)) {
-      for (int j = j0; j < j0 + L::W / S; ++j) vout[u.lane + j * L::N] = T(0);
-    }
-    if constexpr (PATCH) {
-      galp_own_exceptions<T>(a, u.vec, u.lane, j0, L::W / S,
-                             [&](unsigned pos, T e) { store<T, NT>(vout + pos, e); });
+      for (int j = j0; j < j0 + NJ; ++j) vout[u.lane + j * L::N] = T(0);
     }
   }
 }
