@@ -231,7 +231,7 @@ def copy_bandwidth(torch, dev, stream):
 
 
 class CacheFlush:
-    """Evicts the 256 MB Infinity Cache and the L2s before a timed launch by
+    """Evicts the 256 MB Infinity Cache and the L2s around a timed launch by
     reading a 1 GiB scratch buffer, so a launch repeated on one column reads
     it from HBM, as a first launch would. A read leaves clean lines behind; a
     write would leave dirty ones whose write-back lands in the timed launch."""
@@ -239,25 +239,42 @@ class CacheFlush:
     def __init__(self, torch, dev):
         self.torch = torch
         self.buf = torch.ones(1 << 27, dtype=torch.int64, device=dev)
+        self.clean_ms = None
 
     def __call__(self):
         self.torch.sum(self.buf)
 
 
-def cold_launch_ms(torch, stream, fn, reps, flush):
-    """Mean duration of `reps` launches of fn, each after a cache flush and
-    bracketed by its own events on `stream`."""
-    fn()
-    torch.cuda.synchronize()
+def _flushed_events_ms(torch, stream, body, reps, flush):
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(reps)]
     for a, b in evs:
         flush()
         a.record(stream)
-        fn()
+        body()
         b.record(stream)
     torch.cuda.synchronize()
     return sum(a.elapsed_time(b) for a, b in evs) / reps
+
+
+def cold_launch_ms(torch, stream, fn, reps, flush):
+    """Mean sustained cost of `reps` launches of fn on `stream`, each after a
+    cache flush: the launch plus a second flush inside its events, less what
+    that flush takes on clean caches. A kernel with plain stores ends with up
+    to 256 MB of dirty output lines in the Infinity Cache; their write-back is
+    part of its HBM traffic but lands in whatever runs next, so the launch
+    alone would not be charged for it (back-to-back launches, as in the timed
+    sweep, are charged in sequence)."""
+    if flush.clean_ms is None:
+        flush.clean_ms = _flushed_events_ms(torch, stream, flush, reps, flush)
+    fn()
+    torch.cuda.synchronize()
+
+    def body():
+        fn()
+        flush()
+
+    return _flushed_events_ms(torch, stream, body, reps, flush) - flush.clean_ms
 
 
 def time_launches(torch, stream, fn, reps):

@@ -45,23 +45,41 @@ class _Flush:
 _flush = None
 
 
-def _time(fn, reps=10):
-    """Mean duration of `reps` launches, each after a cache flush and
-    bracketed by its own events."""
-    global _flush
-    if _flush is None:
-        _flush = _Flush()
-    fn()
-    torch.cuda.synchronize()
+_flush_ms = None
+
+
+def _events(body, reps):
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(reps)]
     for a, b in evs:
         _flush()
         a.record()
-        fn()
+        body()
         b.record()
     torch.cuda.synchronize()
     return sum(a.elapsed_time(b) for a, b in evs) / reps
+
+
+def _time(fn, reps=10):
+    """Mean cost of `reps` launches, each after a cache flush: the launch plus
+    a second flush inside the events, less what that flush takes on clean
+    caches. A variant with plain stores leaves up to the Infinity Cache's
+    256 MB of dirty output lines, whose write-back lands in whatever runs
+    next; timing the launch alone would not charge it. (Measured in the
+    bit-width sweep: a non-temporal-store kernel after a plain-store one on
+    the same output took 277 us instead of 205 us.)"""
+    global _flush, _flush_ms
+    if _flush is None:
+        _flush = _Flush()
+        _flush_ms = _events(_flush, reps)
+    fn()
+    torch.cuda.synchronize()
+
+    def body():
+        fn()
+        _flush()
+
+    return _events(body, reps) - _flush_ms
 
 
 TIMES: dict = {}  # (encoding, bit width) -> {variant name: ms}, filled by the tuners
