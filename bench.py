@@ -257,18 +257,22 @@ def _flushed_events_ms(torch, stream, body, reps, flush):
     return sum(a.elapsed_time(b) for a, b in evs) / reps
 
 
-def cold_launch_ms(torch, stream, fn, reps, flush):
-    """Mean sustained cost of `reps` launches of fn on `stream`, each after a
-    cache flush: the launch plus a second flush inside its events, less what
-    that flush takes on clean caches. A kernel with plain stores ends with up
-    to 256 MB of dirty output lines in the Infinity Cache; their write-back is
-    part of its HBM traffic but lands in whatever runs next, so the launch
-    alone would not be charged for it (back-to-back launches, as in the timed
-    sweep, are charged in sequence)."""
-    if flush.clean_ms is None:
-        flush.clean_ms = _flushed_events_ms(torch, stream, flush, reps, flush)
+def cold_launch_ms(torch, stream, fn, reps, flush, writes=True):
+    """Mean cost of `reps` launches of fn on `stream`, each after a cache
+    flush. For work that writes (writes=True) the sustained cost: the launch
+    plus a second flush inside its events, less what that flush takes on
+    clean caches. A kernel with plain stores ends with up to 256 MB of dirty
+    output lines in the Infinity Cache; their write-back is part of its HBM
+    traffic but lands in whatever runs next, so the launch alone would not be
+    charged for it (back-to-back launches, as in the timed sweep, are charged
+    in sequence). Read-only work leaves nothing to write back and is timed
+    alone between its events."""
     fn()
     torch.cuda.synchronize()
+    if not writes:
+        return _flushed_events_ms(torch, stream, fn, reps, flush)
+    if flush.clean_ms is None:
+        flush.clean_ms = _flushed_events_ms(torch, stream, flush, reps, flush)
 
     def body():
         fn()
@@ -332,7 +336,8 @@ def c1_extras(args, torch, A, D, E, dev, stream, r, flush):
                                       "hbm_frac": round(algo / ms / 1e6 / HBM_PEAK_GBS, 4)}
 
     ms = cold_launch_ms(torch, stream, lambda: D.query_filter_sum(col, A.PRED_ALL, result=r,
-                                                                  stream=stream), 20, flush)
+                                                                  stream=stream), 20, flush,
+                        writes=False)
     got_sum = int(r.cpu().numpy()[1])
     res["c1_u32_w13_2^25_sum"] = {"us": round(ms * 1e3, 2),
                                    "decompressed_equiv_GBps": round(nbytes / ms / 1e6, 1),
@@ -352,7 +357,7 @@ def filter_sum_extras(args, torch, A, D, E, dev, stream, r, flush):
             pk, bb, oo, ss = E.synthetic_ffor_column_device(n_vecs, bw, width, seed=7 + bw, device=dev)
             col = D.DeviceFFORColumn(nvals, pk, bb, oo, ss, width)
             ms = cold_launch_ms(torch, stream, lambda: D.query_filter_sum(
-                col, A.PRED_OFFSET_LOWER_HALF, result=r, stream=stream), 5, flush)
+                col, A.PRED_OFFSET_LOWER_HALF, result=r, stream=stream), 5, flush, writes=False)
             read = pk.numel() * pk.element_size() + n_vecs * (1 + 8 + width // 8)
             fs.append({"dtype": f"u{width}", "bw": bw, "us": round(ms * 1e3, 1),
                        "decompressed_equiv_GBps": round(nvals * width / 8 / ms / 1e6, 1),
