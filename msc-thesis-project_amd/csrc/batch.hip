@@ -53,11 +53,17 @@ __device__ __forceinline__ void batch_ffor(const BatchColumn *c, uint64_t w0, in
     ffor_decompress_unit<UT, 1, NT>(a, static_cast<UT *>(c->out), w0 + wib + 4 * k);
 }
 
+// G-ALP columns take the per-thread decoder with register patching (no LDS,
+// no barriers); standard-layout ones keep the per-thread scan of the list.
 template <typename T, int LAYOUT, bool NT>
 __device__ __forceinline__ void batch_alp(const BatchColumn *c, uint64_t w0, int wib) {
   const AlpArgs<T> &a = *reinterpret_cast<const AlpArgs<T> *>(&c->u);
-  for (int k = 0; k < TASK_WAVES / 4; ++k)
-    alp_decompress_unit_scan<T, NT, LAYOUT>(a, static_cast<T *>(c->out), w0 + wib + 4 * k);
+  for (int k = 0; k < TASK_WAVES / 4; ++k) {
+    if constexpr (LAYOUT == ALP_GALP)
+      alp_decompress_unit<T, 1, NT, ALP_GALP, true>(a, static_cast<T *>(c->out), w0 + wib + 4 * k);
+    else
+      alp_decompress_unit_scan<T, NT, LAYOUT>(a, static_cast<T *>(c->out), w0 + wib + 4 * k);
+  }
 }
 
 template <typename T, bool NT>
