@@ -85,7 +85,7 @@ int main(int argc, char **argv) {
   uint64_t h[6];
   if (!f || std::fread(h, 8, 6, f) != 6) return 1;
   if (h[0] != sizeof(EMU_T) * 8 || h[4] != EMU_GALP) {
-    std::fprintf(stderr, "column of width %llu, galp %llu: not this build's\n",
+    std::fprintf(stderr, "column of width %llu, galp %llu: not the element type and layout this driver decodes\n",
                  (unsigned long long)h[0], (unsigned long long)h[4]);
     return 1;
   }
