@@ -544,10 +544,34 @@ def cpu_baseline(args):
         if time.perf_counter() - t0 > 10.0:
             break
     dt = time.perf_counter() - t0
+    # All cores: the same columns spread over a thread pool (the ctypes call
+    # releases the GIL, so the decodes run in parallel), at most 16 threads:
+    # the CPU share a one-GPU job gets.
+    from concurrent.futures import ThreadPoolExecutor
+    n_threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    mt_passes = 0
+    with ThreadPoolExecutor(n_threads) as pool:
+        t1 = time.perf_counter()
+        while True:
+            list(pool.map(lambda c: O.ffor_column_decode(c, use_ref=use_ref), samples))
+            mt_passes += 1
+            if time.perf_counter() - t1 > 5.0:
+                break
+        mt_dt = time.perf_counter() - t1
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(dec_bytes * passes / dt / 1e9, 3), "unit": "GB/s", "cores": 1,
             "kind": kind,
             "sample": f"same sweep (u32 w=0..32, u64 w=0..64) at 2^20 values per width, "
-                      f"{passes} passes, single thread, fls::unffor per vector"}
+                      f"{passes} passes, single thread, fls::unffor per vector",
+            "all_cores": {"value": round(dec_bytes * mt_passes / mt_dt / 1e9, 3), "unit": "GB/s",
+                          "cores": n_threads,
+                          "sample": f"the same columns over {n_threads} threads, {mt_passes} passes"},
+            "cpu_model": model}
 
 
 if __name__ == "__main__":
