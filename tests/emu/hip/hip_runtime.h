@@ -7,7 +7,8 @@
 // threads (every shuffle in these kernels stays inside one vector's
 // N_LANES-thread group), readfirstlane is the identity and the ballot
 // answers with a fixed mode: 0 sends every lane down the uniform-width path
-// with its own width, 1 down the generic mixed-width path.
+// with its own width, 1 down the generic mixed-width path. A persistent grid
+// drains its queue in the first block.
 #pragma once
 #include <barrier>
 #include <climits>
@@ -129,3 +130,42 @@ inline void emu_launch(K kernel, dim3 grid, dim3 block, A... args) {
   }
 }
 #define hipLaunchKernelGGL(k, g, b, sh, st, ...) emu_launch(k, dim3(g), dim3(b), __VA_ARGS__)
+
+// The batch kernel's plan and queue: device memory is host memory here,
+// global atomics are host atomics, graph capture is unavailable.
+enum hipMemcpyKind { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2 };
+template <class T>
+inline hipError_t hipMalloc(T **p, size_t n) {
+  *p = static_cast<T *>(std::malloc(n ? n : 1));
+  return *p ? hipSuccess : 1;
+}
+inline hipError_t hipFree(void *p) {
+  std::free(p);
+  return hipSuccess;
+}
+inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) {
+  std::memcpy(d, s, n);
+  return hipSuccess;
+}
+inline hipError_t hipMemset(void *d, int v, size_t n) {
+  std::memset(d, v, n);
+  return hipSuccess;
+}
+inline unsigned long long atomicAdd(unsigned long long *p, unsigned long long v) {
+  return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST);
+}
+inline void __threadfence() { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
+#define __HIP_MEMORY_SCOPE_AGENT 0
+#define __hip_atomic_store(p, v, order, scope) __atomic_store_n((p), (v), __ATOMIC_SEQ_CST)
+#define amdgpu_waves_per_eu(...)
+typedef struct ihipGraph_t *hipGraph_t;
+typedef struct ihipGraphExec_t *hipGraphExec_t;
+enum { hipStreamNonBlocking = 1, hipStreamCaptureModeThreadLocal = 1 };
+inline hipError_t hipStreamCreateWithFlags(hipStream_t *, unsigned) { return 1; }
+inline hipError_t hipStreamDestroy(hipStream_t) { return 1; }
+inline hipError_t hipStreamBeginCapture(hipStream_t, int) { return 1; }
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *) { return 1; }
+inline hipError_t hipGraphInstantiate(hipGraphExec_t *, hipGraph_t, void *, char *, size_t) { return 1; }
+inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return 1; }
+inline hipError_t hipGraphDestroy(hipGraph_t) { return 1; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return 1; }

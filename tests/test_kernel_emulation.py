@@ -1,5 +1,6 @@
-"""The device encoder's kernels (csrc/encode.hip) and the ALP decoders
-(csrc/alp_kernels.hpp) run on the CPU through a host emulation of the HIP
+"""The device encoder's kernels (csrc/encode.hip), the ALP decoders
+(csrc/alp_kernels.hpp) and the batched decoder (csrc/batch.hip: plan, queue,
+persistent kernel over every encoding) run on the CPU through a host emulation of the HIP
 features they use (tests/emu/hip/hip_runtime.h: one OS thread per GPU
 thread, block and wave barriers, group shuffles), built with
 AddressSanitizer. Every output buffer has the exact size the library's
@@ -37,7 +38,7 @@ def drivers(tmp_path_factory):
     if cxx is None:
         pytest.skip("no clang++ for the host emulation build")
     out = tmp_path_factory.mktemp("emu")
-    builds = {"encoder_driver": ("encoder_driver", [])}
+    builds = {"encoder_driver": ("encoder_driver", []), "batch_driver": ("batch_driver", [])}
     for t in ("float", "double"):
         for g in (0, 1):
             builds[f"alp_decoder_{t}_{g}"] = ("alp_decoder_driver", [f"-DEMU_T={t}", f"-DEMU_GALP={g}"])
@@ -209,3 +210,61 @@ def test_alp_decoders_emulated(drivers, tmp_path, dtype, galp):
     assert r.returncode == 0, r.stdout + r.stderr[-3000:]
     lines = r.stdout.strip().splitlines()
     assert len(lines) == 16 and all(l.endswith("differences=0") for l in lines), r.stdout
+
+
+BATCH_ENC = {"bp_u32": 0, "ffor_u32": 1, "bp_u64": 2, "ffor_u64": 3, "alp_f32": 4, "alp_f64": 5,
+             "alp_ext_f32": 6, "alp_ext_f64": 7, "alprd_f32": 8, "alprd_f64": 9}
+
+
+def write_batch(path, specs):
+    """The batch driver's input: each column's descriptor arrays as the
+    device columns hold them (an empty array as one zero element)."""
+    from flsgpu import _capi as A
+
+    def arr(fh, a):
+        a = np.ascontiguousarray(a)
+        if a.size == 0:
+            a = np.zeros(1, dtype=a.dtype)
+        fh.write(struct.pack("<Q", a.nbytes))
+        fh.write(a.tobytes())
+
+    with open(path, "wb") as fh:
+        fh.write(struct.pack("<I", len(specs)))
+        for s in specs:
+            enc = BATCH_ENC[s.kind]
+            patcher = A.PATCHERS["prefetch-all" if enc >= 6 else "stateful"] if 4 <= enc <= 7 else 0
+            fh.write(struct.pack("<iQi", enc, len(s.values) // 1024, patcher))
+            c = s.host
+            if enc <= 3:
+                for a in c[:3] + ((c[3],) if enc in (1, 3) else ()):
+                    arr(fh, a)
+            elif enc <= 7:
+                for a in (c.packed, c.bit_widths, c.vector_offsets, c.bases, c.factor_indices,
+                          c.fraction_indices, c.exceptions_offsets, c.exceptions, c.positions,
+                          c.offsets_counts if enc >= 6 else c.counts):
+                    arr(fh, a)
+            else:
+                d = [int(v) for v in c.dictionary] + [0] * (8 - len(c.dictionary))
+                fh.write(struct.pack("<III8H", c.right_bw, c.left_bw, len(c.dictionary), *d))
+                for a in (c.right_packed, c.left_packed, c.exceptions_offsets, c.exceptions,
+                          c.positions, c.counts):
+                    arr(fh, a)
+            arr(fh, s.values)
+
+
+@pytest.mark.parametrize("ballot", [0, 1])
+def test_batch_emulated(drivers, tmp_path, ballot):
+    """The GPU batch test's columns (every encoding, mixed widths, small and
+    ragged) plus a mixed batch: two runs of one plan, every output exact."""
+    import test_batch_gpu as TB
+    from flsgpu import workloads as WL
+    specs = TB.every_encoding(np.random.default_rng(1)) + WL.mixed_batch(seed=2, n_columns=9,
+                                                                         min_log2=10, max_log2=15)
+    src = str(tmp_path / "batch.bin")
+    write_batch(src, specs)
+    r = subprocess.run([drivers["batch_driver"], src], capture_output=True, text=True, timeout=600,
+                       env=dict(ENV, EMU_BALLOT=str(ballot)))
+    assert "AddressSanitizer" not in r.stderr, r.stderr[-4000:]
+    assert r.returncode == 0, r.stdout + r.stderr[-3000:]
+    runs = [l for l in r.stdout.splitlines() if l.startswith("run ")]
+    assert len(runs) == 2 and all(f"0 of {len(specs)} items differ" in l for l in runs), r.stdout
