@@ -8,11 +8,11 @@
 // contraction (built with -ffp-contract=off), no division by 10^e.
 //
 // Exception patching (src/flsgpu/alp.cuh:48-461, CPU decoder.hpp:137-145):
-// every exception is written by the thread that owns its (lane, slot). With
-// the G-ALP layout that thread reads its lane's header and loads its first few
-// exceptions before decoding, storing them in place of the decoded values;
-// further ones, and those of the standard layout (whose list the wave indexes
-// in LDS), it writes after it stored the decoded value at the same address, so
+// every exception is written by the thread that owns its (lane, slot). That
+// thread loads its first few exceptions before decoding (G-ALP: through its
+// lane's header; standard layout: through the wave's LDS index of the
+// vector's list) and stores them in place of the decoded values; further
+// ones it writes after it stored the decoded value at the same address, so
 // the final value is the exception by single-thread program order. The
 // LDS-staged kernels patch the staged values.
 #pragma once
@@ -53,11 +53,13 @@ struct AlpDecompressPolicy {
   }
 };
 
-// Decode with exceptions patched in registers (G-ALP layout): bit j of
-// `emask` marks slot j as an exception whose value is the head of the queue q
-// (filled in slot order), so each value is stored once. RK entries per
-// thread: a few exceptions per lane are typical (2 decimals, 1 % exceptions:
-// 0.6 per f64 lane; f32 columns of 1 decimal: 4.5).
+// Decode with exceptions patched in registers: bit j of `emask` marks slot j
+// as an exception whose value is the head of the queue q (filled in slot
+// order), so each value is stored once. With non-temporal stores an exception
+// stored after the decode rewrites part of a line already on its way to HBM
+// (2^26 doubles, 10 exceptions per vector, standard layout: 146 -> 133 us).
+// RK entries per thread: a few exceptions per lane are typical (2 decimals,
+// 1 % exceptions: 0.6 per f64 lane; f32 columns of 1 decimal: 4.5).
 template <typename T>
 constexpr int alp_reg_k() { return sizeof(T) == 8 ? 4 : 8; }
 
@@ -314,18 +316,31 @@ __device__ __forceinline__ void alp_decompress_unit(const AlpArgs<T> &a, T *__re
     idx.build(a, wave / S);
     if (u.vec >= a.ffor.n_vecs) return;
     T *vout = out + u.vec * VALUES_PER_VECTOR;
-    // Patched after the decode: register patching (as for G-ALP below) was
-    // slower here, the LDS index build before the decode dominates
-    // (2^26 doubles, 10 exceptions per vector: 134 -> 137 us).
-    AlpDecompressPolicy<T, NT> pol{a.ffor.packed + a.ffor.vector_offsets[u.vec] + u.lane,
-                                   vout + u.lane, a.ffor.bases[u.vec],
-                                   alp_factor<T>(a.factor_indices[u.vec]),
-                                   alp_fraction<T>(a.fraction_indices[u.vec])};
+    const int vv = int((threadIdx.x % WAVE) / L::N);
+    AlpRegPatchPolicy<T, NT> pol{a.ffor.packed + a.ffor.vector_offsets[u.vec] + u.lane,
+                                 vout + u.lane, a.ffor.bases[u.vec],
+                                 alp_factor<T>(a.factor_indices[u.vec]),
+                                 alp_fraction<T>(a.fraction_indices[u.vec]), 0, {}};
+    // the lane's first RK exception slots of the range go to the queue (list
+    // index = rank in the position map), the rest are stored after the decode
+    const UT lm = idx.slots(vv, u.lane, j0, NJ);
+    UT rest = lm;
+    const uint64_t eo = lm ? a.exceptions_offsets[u.vec] : 0;
+#pragma unroll
+    for (int i = 0; i < pol.RK; ++i) {
+      pol.q[i] = T(0);
+      if (rest) {
+        const int j = int(__builtin_ctzll(uint64_t(rest)));
+        rest &= UT(rest - 1);
+        pol.emask |= uint64_t(1) << j;
+        pol.q[i] = a.exceptions[eo + idx.rank(vv, unsigned(u.lane + L::N * j))];
+      }
+    }
     if (!dispatch_bw<UT, S>(pol, a.ffor.bit_widths[u.vec], u.part)) {
       for (int j = j0; j < j0 + NJ; ++j) vout[u.lane + j * L::N] = T(0);
+      rest = lm;  // nothing was patched
     }
-    idx.own(a, u.vec, int((threadIdx.x % WAVE) / L::N), u.lane, j0, NJ,
-            [&](unsigned pos, T e) { store<T, NT>(vout + pos, e); });
+    idx.own_slots(a, u.vec, vv, u.lane, rest, [&](unsigned pos, T e) { store<T, NT>(vout + pos, e); });
   } else if constexpr (PATCH) {
     if (u.vec >= a.ffor.n_vecs) return;
     T *vout = out + u.vec * VALUES_PER_VECTOR;
