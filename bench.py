@@ -487,6 +487,15 @@ def alp_extras(torch, dev, stream, flush):
                 "exceptions_per_vector": round(len(col.exceptions) / col.n_vecs, 2),
                 "compression_ratio": round(c.compression_ratio(extended=name == "galp"), 3),
                 "host_encode_s": round(enc_s, 2)}
+            # "any value == v" over the column (the reference's decompression_query),
+            # v absent so the whole column is scanned; read-only
+            absent = 123456.789 if width == 64 else 12345.678
+            found = torch.empty(1, dtype=torch.uint8, device=dev)
+            qms = cold_launch_ms(torch, stream, lambda: AL.query_any(d, absent, found, stream=stream),
+                                 10, flush, writes=False)
+            res[f"c3_{name}_f{width}_query_any"] = {
+                "us": round(qms * 1e3, 1), "hbm_frac": round(read / qms / 1e6 / HBM_PEAK_GBS, 4),
+                "found": int(found.item()), "expected": int(np.any(x == x.dtype.type(absent)))}
             del d, out
         del ref
         torch.cuda.empty_cache()

@@ -17,6 +17,8 @@
 // LDS-staged kernels patch the staged values.
 #pragma once
 
+#include <algorithm>
+
 #include "alp_impl.hpp"
 #include "kernels.hpp"
 #include "stage.hpp"
@@ -552,19 +554,77 @@ __device__ __forceinline__ StdIndex<T> wave_std_index() {
   }
 }
 
+// "Any value == magic" over a patched column. The patched column holds the
+// exception values at their positions and the decoded values elsewhere, so
+// the answer is: some exception value equals magic, or some decoded value
+// at a position that is not an exception does. The first part reads the
+// exception values only (the vector's threads together, coalesced); the
+// positions are read only by a lane with a decoded match, to drop matches at
+// exception positions, which is rare. (Building the standard layout's LDS
+// index for every vector, as the full patch does, made the query slower than
+// the decode: 2^26 doubles, 91 us.) A position listed twice counts each of
+// its exception values.
+// Threads per lane of the query: two for doubles (the 64-slot decode and
+// compare otherwise needs ~184 VGPRs), one for floats.
+template <typename T>
+constexpr int alp_any_split() { return sizeof(T) == 8 ? 2 : 1; }
+
 template <typename T, int LAYOUT, bool PATCH>
 __global__ __launch_bounds__(BLOCK) void k_alp_any(AlpArgs<T> a, T magic, uint64_t n_waves,
                                                   Scratch s) {
   using UT = typename uint_of<T>::type;
+  using L = lanes<UT>;
+  constexpr int S = alp_any_split<T>();
+  constexpr int NJ = L::W / S;
+  constexpr uint64_t FULL = NJ == 64 ? ~uint64_t(0) : ((uint64_t(1) << NJ) - 1);
   bool found = false;
   const uint64_t stride = uint64_t(gridDim.x) * (BLOCK / WAVE);
-  StdIndex<T> idx = wave_std_index<T, LAYOUT, PATCH>();
   for (uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE; wave < n_waves;
        wave += stride) {
-    const UnitMap<UT, 1> u = map_unit<UT, 1>(wave);
-    if constexpr (PATCH && LAYOUT == ALP_STD) idx.build(a, wave);
-    if (u.vec < a.ffor.n_vecs)
-      found |= alp_lane_match<T, LAYOUT, PATCH>(a, u.vec, u.lane, magic, &idx) != 0;
+    const UnitMap<UT, S> u = map_unit<UT, S>(wave);
+    if (u.vec >= a.ffor.n_vecs) continue;
+    const int j0 = u.part * NJ;
+    AlpMatchPolicy<T> pol{a.ffor.packed + a.ffor.vector_offsets[u.vec] + u.lane,
+                          a.ffor.bases[u.vec], alp_factor<T>(a.factor_indices[u.vec]),
+                          alp_fraction<T>(a.fraction_indices[u.vec]), magic, 0};
+    if (!dispatch_bw<UT, S>(pol, a.ffor.bit_widths[u.vec], u.part))
+      pol.mask = T(0) == magic ? FULL : 0;
+    uint64_t m = pol.mask;  // bit (j - j0): slot j decodes to magic
+    if constexpr (PATCH) {
+      const uint64_t eo = a.exceptions_offsets[u.vec];
+      unsigned count, first, k0, step;
+      if constexpr (LAYOUT == ALP_GALP) {
+        const unsigned oc = a.offsets_counts[u.vec * L::N + u.lane];
+        count = oc >> 10;
+        first = oc & 0x3FFu;
+        k0 = 0;  // the lane's own list
+        step = 1;
+      } else {
+        count = a.counts[u.vec];
+        first = 0;
+        k0 = unsigned(u.lane);  // the vector's list, shared by its lanes
+        step = unsigned(L::N);
+      }
+      if (u.part == 0) {  // the exception values, ALP_PF loads in flight
+        for (unsigned k = k0; k < count; k += ALP_PF * step) {
+          T e[ALP_PF];
+#pragma unroll
+          for (int i = 0; i < ALP_PF; ++i)
+            e[i] = k + i * step < count ? a.exceptions[eo + first + k + i * step] : T(0);
+#pragma unroll
+          for (int i = 0; i < ALP_PF; ++i) found |= k + i * step < count && e[i] == magic;
+        }
+      }
+      if (m) {
+        for (unsigned k = 0; k < count; ++k) {
+          const unsigned pos = unsigned(a.positions[eo + first + k]) & 1023u;
+          const int slot = int(pos / unsigned(L::N));
+          if (int(pos % unsigned(L::N)) == u.lane && slot >= j0 && slot < j0 + NJ)
+            m &= ~(uint64_t(1) << (slot - j0));
+        }
+      }
+    }
+    found |= m != 0;
   }
   block_partial2<RedOp::Or>(found, 0, s.partials);
 }
@@ -624,8 +684,10 @@ int AlpAny<T, LAYOUT, PATCH>::launch(const AlpArgs<T> &a, T magic, uint8_t *d_fo
   Scratch s;
   int rc = get_scratch(&s);
   if (rc) return rc;
-  const uint64_t n_waves = n_waves_for<UT, 1>(a.ffor.n_vecs);
-  const unsigned grid = reduce_grid(n_waves);
+  const uint64_t n_waves = n_waves_for<UT, alp_any_split<T>()>(a.ffor.n_vecs);
+  // a CU-filling grid whose waves walk several units each (2^26 doubles:
+  // 56 us at 2048 blocks, 61 us with one unit per wave)
+  const unsigned grid = std::min(reduce_grid(n_waves), 2048u);
   hipLaunchKernelGGL((k_alp_any<T, LAYOUT, PATCH>), dim3(grid), dim3(BLOCK), 0, stream, a, magic,
                      n_waves, s);
   if ((rc = check_hip(hipGetLastError(), "k_alp_any launch"))) return rc;
