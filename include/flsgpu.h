@@ -18,10 +18,22 @@
  * padding is needed after the packed array or the exception arrays.
  *
  * Reductions (query_any, filter_sum, compute, multi-column) are two launches
- * on `stream`: the work kernel leaves one partial per block in a small
- * per-device scratch area owned by the library, and a one-block kernel
- * writes the final result to the caller's device pointer. Concurrent calls
- * of these reduction entry points on different streams of one device race.
+ * on `stream`: the work kernel leaves one partial per block in a scratch
+ * area owned by the library, and a one-block kernel writes the final result
+ * to the caller's device pointer. Each (device, stream) pair gets its own
+ * 1 MiB area on its first reduction, so reductions on different streams may
+ * run concurrently; calls on one stream are ordered by the stream. A
+ * reduction captured into a hipGraph keeps the area of the stream it was
+ * captured on: replays of that graph must not overlap reductions issued on
+ * that stream. flsgpu_release_stream frees a stream's area.
+ *
+ * Exception lists: positions are 0..1023 within the vector and strictly
+ * increasing within each vector's list (standard layout, ALP-RD) or within
+ * each lane's list (G-ALP, whose lane l holds only positions p with
+ * p % N_LANES == l), as the reference's encoder and G-ALP conversion write
+ * them (src/alp/encoder.hpp:352-358, src/flsgpu/structs.cuh:231-306). The
+ * decoders rely on it; flsgpu_column_upload and flsgpu_file_load reject
+ * other lists with FLSGPU_EINVAL.
  */
 #ifndef FLSGPU_H
 #define FLSGPU_H
@@ -163,6 +175,11 @@ int flsgpu_free(void *d_ptr);
 /* Waits for `stream` (NULL: the whole device, as cudaDeviceSynchronize in
  * kernels::host::*, src/engine/kernels.cuh:106,128,148). */
 int flsgpu_synchronize(flsgpu_stream_t stream);
+/* Waits for `stream` and frees its reduction scratch area on the current
+ * device (no reference counterpart; the reference has one default stream).
+ * A later reduction on the stream allocates a new one. At most 256 streams
+ * per device hold an area at once. */
+int flsgpu_release_stream(flsgpu_stream_t stream);
 
 /* ---- decompress: kernels::host::decompress_column (src/engine/kernels.cuh:94-112),
  *      bindings::decompress_column (src/generated-bindings/kernel-bindings.cuh:11-16,37-175) */
@@ -215,7 +232,15 @@ int flsgpu_query_any_alp_ext_f64(const flsgpu_alp_column *col, double magic, uin
 
 /* ---- fused decode + filter + sum (no reference counterpart: the reference
  *      query kernel only answers "any == magic"). d_result[0] = number of
- *      rows passing, d_result[1] = sum (mod 2^64) of the passing values. */
+ *      rows passing, d_result[1] = sum (mod 2^64) of the passing values.
+ *      For a BP column the FOR offset of FLSGPU_PRED_OFFSET_LOWER_HALF is the
+ *      value itself (base 0). */
+int flsgpu_query_filter_sum_bp_u32(const flsgpu_bp_column *col, int predicate, uint64_t lo,
+                                   uint64_t hi, uint64_t *d_result, int variant,
+                                   flsgpu_stream_t stream);
+int flsgpu_query_filter_sum_bp_u64(const flsgpu_bp_column *col, int predicate, uint64_t lo,
+                                   uint64_t hi, uint64_t *d_result, int variant,
+                                   flsgpu_stream_t stream);
 int flsgpu_query_filter_sum_ffor_u32(const flsgpu_ffor_column *col, int predicate, uint64_t lo,
                                      uint64_t hi, uint64_t *d_result, int variant,
                                      flsgpu_stream_t stream);
@@ -226,6 +251,10 @@ int flsgpu_query_filter_sum_ffor_u64(const flsgpu_ffor_column *col, int predicat
 /* ---- synthetic ALU load: kernels::host::compute_column (kernels.cuh:135-153,
  *      device :56-89): per value n_repetitions of *=3, <<=3, +=3, >>=3, &=0,
  *      then "any == 1"; *d_result is written with 0 or 1. */
+int flsgpu_compute_bp_u32(const flsgpu_bp_column *col, uint32_t n_repetitions,
+                          uint8_t *d_result, int variant, flsgpu_stream_t stream);
+int flsgpu_compute_bp_u64(const flsgpu_bp_column *col, uint32_t n_repetitions,
+                          uint8_t *d_result, int variant, flsgpu_stream_t stream);
 int flsgpu_compute_ffor_u32(const flsgpu_ffor_column *col, uint32_t n_repetitions,
                             uint8_t *d_result, int variant, flsgpu_stream_t stream);
 int flsgpu_compute_ffor_u64(const flsgpu_ffor_column *col, uint32_t n_repetitions,

@@ -30,6 +30,39 @@ struct MultiAlpArgs {
   uint32_t n_cols;
 };
 
+// Checks an ALP descriptor of the given layout and converts it to kernel
+// arguments.
+template <typename T>
+int make_alp_args(const flsgpu_alp_column *c, int layout, AlpArgs<T> *a) {
+  using UT = typename uint_of<T>::type;
+  if (!c) return set_error(FLSGPU_EINVAL, "null column");
+  const flsgpu_bp_column &bp = c->ffor.bp;
+  if (bp.n_values != bp.n_vecs * VALUES_PER_VECTOR)
+    return set_error(FLSGPU_EINVAL, "n_values (%llu) must equal n_vecs * 1024 (n_vecs = %llu)",
+                     (unsigned long long)bp.n_values, (unsigned long long)bp.n_vecs);
+  if (bp.n_vecs > 0) {
+    if (!bp.packed_array || !bp.bit_widths || !bp.vector_offsets || !c->ffor.bases ||
+        !c->factor_indices || !c->fraction_indices || !c->exceptions_offsets)
+      return set_error(FLSGPU_EINVAL, "null device pointer in ALP column");
+    if (layout == ALP_STD && !c->counts)
+      return set_error(FLSGPU_EINVAL, "ALPColumn needs counts");
+    if (layout == ALP_GALP && !c->offsets_counts)
+      return set_error(FLSGPU_EINVAL, "ALPExtendedColumn needs offsets_counts");
+    if (c->n_exceptions > 0 && (!c->exceptions || !c->positions))
+      return set_error(FLSGPU_EINVAL, "null exception arrays");
+  }
+  a->ffor = FforArgs<UT>{static_cast<const UT *>(bp.packed_array), bp.bit_widths,
+                         bp.vector_offsets, static_cast<const UT *>(c->ffor.bases), bp.n_vecs};
+  a->factor_indices = c->factor_indices;
+  a->fraction_indices = c->fraction_indices;
+  a->exceptions_offsets = c->exceptions_offsets;
+  a->exceptions = static_cast<const T *>(c->exceptions);
+  a->positions = c->positions;
+  a->counts = c->counts;
+  a->offsets_counts = c->offsets_counts;
+  return FLSGPU_OK;
+}
+
 template <typename T, int S, bool NT, int LAYOUT, bool PATCH>
 struct AlpDecompress {
   static int launch(const AlpArgs<T> &a, T *out, hipStream_t stream);

@@ -682,7 +682,7 @@ int AlpAny<T, LAYOUT, PATCH>::launch(const AlpArgs<T> &a, T magic, uint8_t *d_fo
                                      hipStream_t stream) {
   using UT = typename uint_of<T>::type;
   Scratch s;
-  int rc = get_scratch(&s);
+  int rc = get_scratch(stream, &s);
   if (rc) return rc;
   const uint64_t n_waves = n_waves_for<UT, alp_any_split<T>()>(a.ffor.n_vecs);
   // a CU-filling grid whose waves walk several units each (2^26 doubles:
@@ -699,7 +699,7 @@ int AlpMulti<T, LAYOUT, PATCH>::launch(const MultiAlpArgs<T> &a, T value, uint8_
                                        hipStream_t stream) {
   using UT = typename uint_of<T>::type;
   Scratch s;
-  int rc = get_scratch(&s);
+  int rc = get_scratch(stream, &s);
   if (rc) return rc;
   const uint64_t n_waves = n_waves_for<UT, 1>(a.col[0].ffor.n_vecs);
   const unsigned grid = reduce_grid(n_waves);

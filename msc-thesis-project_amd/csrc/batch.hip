@@ -158,57 +158,6 @@ uint64_t waves_of(uint32_t enc, uint64_t n_vecs) {
   }
 }
 
-template <typename UT>
-int ffor_args(const flsgpu_bp_column *bp, const void *bases, FforArgs<UT> *a) {
-  if (bp->n_values != bp->n_vecs * VALUES_PER_VECTOR)
-    return set_error(FLSGPU_EINVAL, "batch: n_values must equal n_vecs * 1024");
-  if (bp->n_vecs && (!bp->packed_array || !bp->bit_widths || !bp->vector_offsets))
-    return set_error(FLSGPU_EINVAL, "batch: null device pointer in column");
-  *a = FforArgs<UT>{static_cast<const UT *>(bp->packed_array), bp->bit_widths, bp->vector_offsets,
-                    static_cast<const UT *>(bases), bp->n_vecs};
-  return FLSGPU_OK;
-}
-
-template <typename T>
-int alp_args(const flsgpu_alp_column *c, bool ext, AlpArgs<T> *a) {
-  using UT = typename uint_of<T>::type;
-  int rc = ffor_args<UT>(&c->ffor.bp, c->ffor.bases, &a->ffor);
-  if (rc) return rc;
-  if (c->ffor.bp.n_vecs && (!c->ffor.bases || !c->factor_indices || !c->fraction_indices ||
-                            !c->exceptions_offsets || (ext ? !c->offsets_counts : !c->counts)))
-    return set_error(FLSGPU_EINVAL, "batch: null device pointer in ALP column");
-  a->factor_indices = c->factor_indices;
-  a->fraction_indices = c->fraction_indices;
-  a->exceptions_offsets = c->exceptions_offsets;
-  a->exceptions = static_cast<const T *>(c->exceptions);
-  a->positions = c->positions;
-  a->counts = c->counts;
-  a->offsets_counts = c->offsets_counts;
-  return FLSGPU_OK;
-}
-
-template <typename T>
-int rd_args(const flsgpu_alprd_column *c, AlprdArgs<T> *a) {
-  constexpr unsigned W = sizeof(T) * 8;
-  if (c->n_values != c->n_vecs * VALUES_PER_VECTOR)
-    return set_error(FLSGPU_EINVAL, "batch: n_values must equal n_vecs * 1024");
-  if (c->right_bit_width < W - 16 || c->right_bit_width > W - 1 || c->left_bit_width < 1 ||
-      c->left_bit_width > 3 || c->dictionary_size < 1 || c->dictionary_size > 8)
-    return set_error(FLSGPU_EINVAL, "batch: invalid ALP-RD parameters");
-  a->n_vecs = c->n_vecs;
-  a->right_bw = c->right_bit_width;
-  a->left_bw = c->left_bit_width;
-  a->dict_size = c->dictionary_size;
-  for (int i = 0; i < 8; ++i) a->dict[i] = c->dictionary[i];
-  a->right = static_cast<const typename uint_of<T>::type *>(c->right_packed);
-  a->left = c->left_packed;
-  a->exceptions_offsets = c->exceptions_offsets;
-  a->exceptions = c->exceptions;
-  a->positions = c->positions;
-  a->counts = c->counts;
-  return FLSGPU_OK;
-}
-
 void destroy(flsgpu_batch_plan *p) {
   if (!p) return;
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
@@ -242,45 +191,47 @@ int flsgpu_batch_plan_create(const flsgpu_batch_item *items, uint32_t n_items, i
     uint64_t n_vecs = 0;
     switch (it.encoding) {
       case FLSGPU_ENC_BP_U32:
-        rc = ffor_args<uint32_t>(static_cast<const flsgpu_bp_column *>(it.column), nullptr, &c.u.f32);
+        rc = make_ffor_args<uint32_t>(static_cast<const flsgpu_bp_column *>(it.column), nullptr, &c.u.f32);
         n_vecs = c.u.f32.n_vecs;
         break;
       case FLSGPU_ENC_BP_U64:
-        rc = ffor_args<uint64_t>(static_cast<const flsgpu_bp_column *>(it.column), nullptr, &c.u.f64);
+        rc = make_ffor_args<uint64_t>(static_cast<const flsgpu_bp_column *>(it.column), nullptr, &c.u.f64);
         n_vecs = c.u.f64.n_vecs;
         break;
       case FLSGPU_ENC_FFOR_U32: {
         auto *f = static_cast<const flsgpu_ffor_column *>(it.column);
-        rc = ffor_args<uint32_t>(&f->bp, f->bases, &c.u.f32);
-        if (!rc && f->bp.n_vecs && !f->bases) rc = set_error(FLSGPU_EINVAL, "batch: null bases");
+        rc = f->bp.n_vecs && !f->bases ? set_error(FLSGPU_EINVAL, "null bases")
+                                       : make_ffor_args<uint32_t>(&f->bp, f->bases, &c.u.f32);
         n_vecs = c.u.f32.n_vecs;
         break;
       }
       case FLSGPU_ENC_FFOR_U64: {
         auto *f = static_cast<const flsgpu_ffor_column *>(it.column);
-        rc = ffor_args<uint64_t>(&f->bp, f->bases, &c.u.f64);
-        if (!rc && f->bp.n_vecs && !f->bases) rc = set_error(FLSGPU_EINVAL, "batch: null bases");
+        rc = f->bp.n_vecs && !f->bases ? set_error(FLSGPU_EINVAL, "null bases")
+                                       : make_ffor_args<uint64_t>(&f->bp, f->bases, &c.u.f64);
         n_vecs = c.u.f64.n_vecs;
         break;
       }
       case FLSGPU_ENC_ALP_F32:
       case FLSGPU_ENC_ALP_EXT_F32:
-        rc = alp_args<float>(static_cast<const flsgpu_alp_column *>(it.column),
-                             it.encoding == FLSGPU_ENC_ALP_EXT_F32, &c.u.a32);
+        rc = make_alp_args<float>(static_cast<const flsgpu_alp_column *>(it.column),
+                                  it.encoding == FLSGPU_ENC_ALP_EXT_F32 ? ALP_GALP : ALP_STD,
+                                  &c.u.a32);
         n_vecs = c.u.a32.ffor.n_vecs;
         break;
       case FLSGPU_ENC_ALP_F64:
       case FLSGPU_ENC_ALP_EXT_F64:
-        rc = alp_args<double>(static_cast<const flsgpu_alp_column *>(it.column),
-                              it.encoding == FLSGPU_ENC_ALP_EXT_F64, &c.u.a64);
+        rc = make_alp_args<double>(static_cast<const flsgpu_alp_column *>(it.column),
+                                   it.encoding == FLSGPU_ENC_ALP_EXT_F64 ? ALP_GALP : ALP_STD,
+                                   &c.u.a64);
         n_vecs = c.u.a64.ffor.n_vecs;
         break;
       case FLSGPU_ENC_ALPRD_F32:
-        rc = rd_args<float>(static_cast<const flsgpu_alprd_column *>(it.column), &c.u.r32);
+        rc = make_rd_args<float>(static_cast<const flsgpu_alprd_column *>(it.column), &c.u.r32);
         n_vecs = c.u.r32.n_vecs;
         break;
       case FLSGPU_ENC_ALPRD_F64:
-        rc = rd_args<double>(static_cast<const flsgpu_alprd_column *>(it.column), &c.u.r64);
+        rc = make_rd_args<double>(static_cast<const flsgpu_alprd_column *>(it.column), &c.u.r64);
         n_vecs = c.u.r64.n_vecs;
         break;
       default:

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <vector>
 
 #include "common.hpp"
 
@@ -33,8 +34,19 @@ int check_hip(hipError_t e, const char *what) {
 }
 
 constexpr int MAX_DEVICES = 64;
+// Reduction areas per device, one per stream that ran a reduction. A stream
+// handle is only a key here: the area of a destroyed stream stays until
+// flsgpu_release_stream, and a new stream that reuses the handle reuses the
+// area (in its own stream order).
+constexpr size_t MAX_STREAM_SCRATCH = 256;
+constexpr size_t SCRATCH_BYTES = sizeof(uint64_t) * 2 * MAX_REDUCE_BLOCKS;
+struct StreamScratch {
+  hipStream_t stream;
+  uint64_t *partials;
+};
 static std::mutex g_mu;
-static Scratch g_scratch[MAX_DEVICES];
+static std::vector<StreamScratch> g_scratch[MAX_DEVICES];
+static bool g_ready[MAX_DEVICES];
 static int g_cus[MAX_DEVICES];
 
 static int current_device(int *dev) {
@@ -45,25 +57,36 @@ static int current_device(int *dev) {
 }
 
 static int init_device_locked(int dev) {
-  if (g_scratch[dev].partials) return FLSGPU_OK;
+  if (g_ready[dev]) return FLSGPU_OK;
   hipDeviceProp_t prop;
   int rc = check_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
   if (rc) return rc;
   g_cus[dev] = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-  void *p = nullptr;
-  const size_t bytes = sizeof(uint64_t) * 2 * MAX_REDUCE_BLOCKS;
-  if ((rc = check_hip(hipMalloc(&p, bytes), "hipMalloc(scratch)"))) return rc;
-  g_scratch[dev].partials = static_cast<uint64_t *>(p);
+  g_ready[dev] = true;
   return FLSGPU_OK;
 }
 
-int get_scratch(Scratch *s) {
+int get_scratch(hipStream_t stream, Scratch *s) {
   int dev;
   int rc = current_device(&dev);
   if (rc) return rc;
   std::lock_guard<std::mutex> lk(g_mu);
   if ((rc = init_device_locked(dev))) return rc;
-  *s = g_scratch[dev];
+  for (const StreamScratch &e : g_scratch[dev]) {
+    if (e.stream == stream) {
+      s->partials = e.partials;
+      return FLSGPU_OK;
+    }
+  }
+  if (g_scratch[dev].size() >= MAX_STREAM_SCRATCH)
+    return set_error(FLSGPU_ENOMEM,
+                     "reductions on more than %zu streams of device %d: release finished "
+                     "streams with flsgpu_release_stream",
+                     MAX_STREAM_SCRATCH, dev);
+  void *p = nullptr;
+  if ((rc = check_hip(hipMalloc(&p, SCRATCH_BYTES), "hipMalloc(scratch)"))) return rc;
+  g_scratch[dev].push_back({stream, static_cast<uint64_t *>(p)});
+  s->partials = static_cast<uint64_t *>(p);
   return FLSGPU_OK;
 }
 
@@ -221,6 +244,25 @@ int flsgpu_download(void *h_ptr, const void *d_ptr, size_t bytes, flsgpu_stream_
                      "hipMemcpyAsync D2H");
   if (rc) return rc;
   return check_hip(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+int flsgpu_release_stream(flsgpu_stream_t stream) {
+  int dev;
+  int rc = current_device(&dev);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto &v = g_scratch[dev];
+  for (size_t i = 0; i < v.size(); ++i) {
+    if (v[i].stream != stream) continue;
+    // the stream's last reduction may still read the area
+    if ((rc = check_hip(stream ? hipStreamSynchronize(stream) : hipDeviceSynchronize(),
+                        "synchronize before release")))
+      return rc;
+    rc = check_hip(hipFree(v[i].partials), "hipFree(scratch)");
+    v.erase(v.begin() + i);
+    return rc;
+  }
+  return FLSGPU_OK;
 }
 
 int flsgpu_synchronize(flsgpu_stream_t stream) {

@@ -161,6 +161,46 @@ int ext_exception_total(const uint64_t *eo, const uint16_t *oc, uint64_t n_vecs,
   return FLSGPU_OK;
 }
 
+// Exception positions must lie in 0..1023 and be strictly increasing within
+// a vector's list (standard layout, ALP-RD) or within a lane's list (G-ALP,
+// where lane l's list may only hold positions p with p % N_LANES == l). The
+// decoders rely on it: the standard-layout patch takes an exception's index
+// in the list as its rank among the vector's exception positions, and each
+// position is written by one owner exactly once. The reference's CPU
+// patcher (src/alp/decoder.hpp:137-145) would apply other lists last-wins,
+// so such lists are rejected rather than decoded differently. Bounds of
+// every list against n_exc must have been checked by the caller.
+int check_positions(const Kind &k, uint64_t nv, const uint64_t *eo, const uint16_t *counts,
+                    const uint16_t *oc, const uint16_t *pos) {
+  const uint64_t nl = uint64_t(1024 / k.width);
+  auto increasing = [&](uint64_t first, uint64_t n, int lane, uint64_t v) -> int {
+    int prev = -1;
+    for (uint64_t i = 0; i < n; ++i) {
+      const int p = pos[first + i];
+      if (p >= 1024 || p <= prev || (lane >= 0 && uint64_t(p) % nl != uint64_t(lane)))
+        return set_error(FLSGPU_EINVAL,
+                         "vector %llu: exception positions must be strictly increasing%s "
+                         "(entry %llu, position %d)",
+                         (unsigned long long)v, lane >= 0 ? " and in their lane" : "",
+                         (unsigned long long)i, p);
+      prev = p;
+    }
+    return FLSGPU_OK;
+  };
+  for (uint64_t v = 0; v < nv; ++v) {
+    int rc;
+    if (k.ext) {
+      for (uint64_t l = 0; l < nl; ++l) {
+        const uint32_t x = oc[v * nl + l];
+        if ((rc = increasing(eo[v] + (x & 0x3FFu), x >> 10, int(l), v))) return rc;
+      }
+    } else if ((rc = increasing(eo[v], counts[v], -1, v))) {
+      return rc;
+    }
+  }
+  return FLSGPU_OK;
+}
+
 int plan_columns(int encoding, const void *column, Writer *w) {
   Kind k;
   if (!kind_of(encoding, &k)) return set_error(FLSGPU_EINVAL, "unknown encoding %d", encoding);
@@ -186,7 +226,13 @@ int plan_columns(int encoding, const void *column, Writer *w) {
     h.rd_dictionary_size = c->dictionary_size;
     memcpy(h.rd_dictionary, c->dictionary, sizeof(h.rd_dictionary));
     uint64_t n_exc = 0;
+    if (c->n_vecs && (!c->exceptions_offsets || !c->counts))
+      return set_error(FLSGPU_EINVAL, "null exception offsets or counts");
     if (c->n_vecs && (rc = exception_total(c->exceptions_offsets, c->counts, c->n_vecs, &n_exc)))
+      return rc;
+    if (n_exc && !c->positions) return set_error(FLSGPU_EINVAL, "null positions");
+    if (c->n_vecs && (rc = check_positions(k, c->n_vecs, c->exceptions_offsets, c->counts,
+                                           nullptr, c->positions)))
       return rc;
     w->add(S_RIGHT_PACKED, wb, c->n_vecs * c->right_bit_width * nl, c->right_packed);
     w->add(S_LEFT_PACKED, 2, c->n_vecs * c->left_bit_width * 64, c->left_packed);
@@ -223,10 +269,16 @@ int plan_columns(int encoding, const void *column, Writer *w) {
     if (a) {
       uint64_t n_exc = 0;
       if (bp->n_vecs) {
+        if (!a->exceptions_offsets || !(k.ext ? a->offsets_counts : a->counts))
+          return set_error(FLSGPU_EINVAL, "null exception offsets or counts");
         rc = k.ext ? ext_exception_total(a->exceptions_offsets, a->offsets_counts, bp->n_vecs,
                                          k.width, &n_exc)
                    : exception_total(a->exceptions_offsets, a->counts, bp->n_vecs, &n_exc);
         if (rc) return rc;
+        if (n_exc && !a->positions) return set_error(FLSGPU_EINVAL, "null positions");
+        if ((rc = check_positions(k, bp->n_vecs, a->exceptions_offsets, a->counts,
+                                  a->offsets_counts, a->positions)))
+          return rc;
       }
       w->add(S_FACTOR_INDICES, 1, bp->n_vecs, a->factor_indices);
       w->add(S_FRACTION_INDICES, 1, bp->n_vecs, a->fraction_indices);
@@ -379,6 +431,7 @@ int parse(FILE *f, Parsed *p) {
             return set_error(FLSGPU_EINVAL, "vector %llu lane %llu: exceptions out of range",
                              (unsigned long long)v, (unsigned long long)l);
         }
+      if ((rc = check_positions(k, nv, eo.data(), nullptr, oc.data(), pos.data()))) return rc;
     } else {
       std::vector<uint16_t> counts;
       if ((rc = read_section(f, h, S_COUNTS, &counts))) return rc;
@@ -386,6 +439,7 @@ int parse(FILE *f, Parsed *p) {
         if (counts[v] > 1024 || eo[v] > n_exc || counts[v] > n_exc - eo[v])
           return set_error(FLSGPU_EINVAL, "vector %llu: exceptions out of range",
                            (unsigned long long)v);
+      if ((rc = check_positions(k, nv, eo.data(), counts.data(), nullptr, pos.data()))) return rc;
     }
   }
   return FLSGPU_OK;

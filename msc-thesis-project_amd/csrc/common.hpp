@@ -119,8 +119,9 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
   return v;
 }
 
-// Per-device scratch owned by the library for cross-block reductions: one
-// partial pair per block of the work kernel.
+// Scratch owned by the library for cross-block reductions: one partial pair
+// per block of the work kernel, one area per (device, stream), so that
+// reductions in flight on different streams never share partials.
 constexpr int MAX_REDUCE_BLOCKS = 65536;
 struct Scratch {
   uint64_t *partials;  // [MAX_REDUCE_BLOCKS * 2]
@@ -129,7 +130,9 @@ struct Scratch {
 // Status helpers (capi.hip)
 int set_error(int code, const char *fmt, ...);
 int check_hip(hipError_t e, const char *what);
-int get_scratch(Scratch *s);
+// The reduction area of `stream` on the current device (allocated on the
+// stream's first reduction, kept until flsgpu_release_stream or exit).
+int get_scratch(hipStream_t stream, Scratch *s);
 int num_cus();
 // Grid for grid-stride reduction kernels: one wave per unit of work, at
 // most MAX_REDUCE_BLOCKS blocks.

@@ -31,19 +31,6 @@ static int resolve_variant(int variant, int dtype_bits, VariantShape *vs) {
 }
 
 template <typename UT>
-static int make_args(const flsgpu_bp_column *bp, const void *bases, FforArgs<UT> *a) {
-  if (!bp) return set_error(FLSGPU_EINVAL, "null column");
-  if (bp->n_values != bp->n_vecs * VALUES_PER_VECTOR)
-    return set_error(FLSGPU_EINVAL, "n_values (%llu) must equal n_vecs * 1024 (n_vecs = %llu)",
-                     (unsigned long long)bp->n_values, (unsigned long long)bp->n_vecs);
-  if (bp->n_vecs > 0 && (!bp->packed_array || !bp->bit_widths || !bp->vector_offsets))
-    return set_error(FLSGPU_EINVAL, "null device pointer in column");
-  *a = FforArgs<UT>{static_cast<const UT *>(bp->packed_array), bp->bit_widths, bp->vector_offsets,
-                    static_cast<const UT *>(bases), bp->n_vecs};
-  return FLSGPU_OK;
-}
-
-template <typename UT>
 static int launch_decompress(const FforArgs<UT> &a, UT *out, int variant, hipStream_t stream) {
   if (!out && a.n_vecs) return set_error(FLSGPU_EINVAL, "null output");
   VariantShape vs;
@@ -112,7 +99,7 @@ static int launch_multi(const flsgpu_ffor_column *cols, uint32_t n_cols, UT valu
   m.n_cols = n_cols;
   for (uint32_t c = 0; c < n_cols; ++c) {
     if (!cols[c].bases && cols[c].bp.n_vecs) return set_error(FLSGPU_EINVAL, "null bases");
-    int rc = make_args<UT>(&cols[c].bp, cols[c].bases, &m.col[c]);
+    int rc = make_ffor_args<UT>(&cols[c].bp, cols[c].bases, &m.col[c]);
     if (rc) return rc;
     if (m.col[c].n_vecs != m.col[0].n_vecs)
       return set_error(FLSGPU_EINVAL, "all columns must have the same number of vectors");
@@ -129,63 +116,77 @@ extern "C" {
 int flsgpu_decompress_bp_u32(const flsgpu_bp_column *col, uint32_t *out, int variant,
                              flsgpu_stream_t stream) {
   FforArgs<uint32_t> a;
-  int rc = make_args<uint32_t>(col, nullptr, &a);
+  int rc = make_ffor_args<uint32_t>(col, nullptr, &a);
   return rc ? rc : launch_decompress(a, out, variant, stream);
 }
 int flsgpu_decompress_bp_u64(const flsgpu_bp_column *col, uint64_t *out, int variant,
                              flsgpu_stream_t stream) {
   FforArgs<uint64_t> a;
-  int rc = make_args<uint64_t>(col, nullptr, &a);
+  int rc = make_ffor_args<uint64_t>(col, nullptr, &a);
   return rc ? rc : launch_decompress(a, out, variant, stream);
 }
 int flsgpu_decompress_ffor_u32(const flsgpu_ffor_column *col, uint32_t *out, int variant,
                                flsgpu_stream_t stream) {
   if (!col || (!col->bases && col->bp.n_vecs)) return set_error(FLSGPU_EINVAL, "null column/bases");
   FforArgs<uint32_t> a;
-  int rc = make_args<uint32_t>(&col->bp, col->bases, &a);
+  int rc = make_ffor_args<uint32_t>(&col->bp, col->bases, &a);
   return rc ? rc : launch_decompress(a, out, variant, stream);
 }
 int flsgpu_decompress_ffor_u64(const flsgpu_ffor_column *col, uint64_t *out, int variant,
                                flsgpu_stream_t stream) {
   if (!col || (!col->bases && col->bp.n_vecs)) return set_error(FLSGPU_EINVAL, "null column/bases");
   FforArgs<uint64_t> a;
-  int rc = make_args<uint64_t>(&col->bp, col->bases, &a);
+  int rc = make_ffor_args<uint64_t>(&col->bp, col->bases, &a);
   return rc ? rc : launch_decompress(a, out, variant, stream);
 }
 
 int flsgpu_query_any_bp_u32(const flsgpu_bp_column *col, uint32_t magic, uint8_t *d_found,
                             int variant, flsgpu_stream_t stream) {
   FforArgs<uint32_t> a;
-  int rc = make_args<uint32_t>(col, nullptr, &a);
+  int rc = make_ffor_args<uint32_t>(col, nullptr, &a);
   return rc ? rc : launch_any(a, magic, d_found, variant, stream);
 }
 int flsgpu_query_any_bp_u64(const flsgpu_bp_column *col, uint64_t magic, uint8_t *d_found,
                             int variant, flsgpu_stream_t stream) {
   FforArgs<uint64_t> a;
-  int rc = make_args<uint64_t>(col, nullptr, &a);
+  int rc = make_ffor_args<uint64_t>(col, nullptr, &a);
   return rc ? rc : launch_any(a, magic, d_found, variant, stream);
 }
 int flsgpu_query_any_ffor_u32(const flsgpu_ffor_column *col, uint32_t magic, uint8_t *d_found,
                               int variant, flsgpu_stream_t stream) {
   if (!col || (!col->bases && col->bp.n_vecs)) return set_error(FLSGPU_EINVAL, "null column/bases");
   FforArgs<uint32_t> a;
-  int rc = make_args<uint32_t>(&col->bp, col->bases, &a);
+  int rc = make_ffor_args<uint32_t>(&col->bp, col->bases, &a);
   return rc ? rc : launch_any(a, magic, d_found, variant, stream);
 }
 int flsgpu_query_any_ffor_u64(const flsgpu_ffor_column *col, uint64_t magic, uint8_t *d_found,
                               int variant, flsgpu_stream_t stream) {
   if (!col || (!col->bases && col->bp.n_vecs)) return set_error(FLSGPU_EINVAL, "null column/bases");
   FforArgs<uint64_t> a;
-  int rc = make_args<uint64_t>(&col->bp, col->bases, &a);
+  int rc = make_ffor_args<uint64_t>(&col->bp, col->bases, &a);
   return rc ? rc : launch_any(a, magic, d_found, variant, stream);
 }
 
+int flsgpu_query_filter_sum_bp_u32(const flsgpu_bp_column *col, int predicate, uint64_t lo,
+                                   uint64_t hi, uint64_t *d_result, int variant,
+                                   flsgpu_stream_t stream) {
+  FforArgs<uint32_t> a;
+  int rc = make_ffor_args<uint32_t>(col, nullptr, &a);
+  return rc ? rc : launch_filter_sum(a, predicate, lo, hi, d_result, variant, stream);
+}
+int flsgpu_query_filter_sum_bp_u64(const flsgpu_bp_column *col, int predicate, uint64_t lo,
+                                   uint64_t hi, uint64_t *d_result, int variant,
+                                   flsgpu_stream_t stream) {
+  FforArgs<uint64_t> a;
+  int rc = make_ffor_args<uint64_t>(col, nullptr, &a);
+  return rc ? rc : launch_filter_sum(a, predicate, lo, hi, d_result, variant, stream);
+}
 int flsgpu_query_filter_sum_ffor_u32(const flsgpu_ffor_column *col, int predicate, uint64_t lo,
                                      uint64_t hi, uint64_t *d_result, int variant,
                                      flsgpu_stream_t stream) {
   if (!col || (!col->bases && col->bp.n_vecs)) return set_error(FLSGPU_EINVAL, "null column/bases");
   FforArgs<uint32_t> a;
-  int rc = make_args<uint32_t>(&col->bp, col->bases, &a);
+  int rc = make_ffor_args<uint32_t>(&col->bp, col->bases, &a);
   return rc ? rc : launch_filter_sum(a, predicate, lo, hi, d_result, variant, stream);
 }
 int flsgpu_query_filter_sum_ffor_u64(const flsgpu_ffor_column *col, int predicate, uint64_t lo,
@@ -193,22 +194,34 @@ int flsgpu_query_filter_sum_ffor_u64(const flsgpu_ffor_column *col, int predicat
                                      flsgpu_stream_t stream) {
   if (!col || (!col->bases && col->bp.n_vecs)) return set_error(FLSGPU_EINVAL, "null column/bases");
   FforArgs<uint64_t> a;
-  int rc = make_args<uint64_t>(&col->bp, col->bases, &a);
+  int rc = make_ffor_args<uint64_t>(&col->bp, col->bases, &a);
   return rc ? rc : launch_filter_sum(a, predicate, lo, hi, d_result, variant, stream);
 }
 
+int flsgpu_compute_bp_u32(const flsgpu_bp_column *col, uint32_t n_repetitions,
+                          uint8_t *d_result, int variant, flsgpu_stream_t stream) {
+  FforArgs<uint32_t> a;
+  int rc = make_ffor_args<uint32_t>(col, nullptr, &a);
+  return rc ? rc : launch_compute(a, n_repetitions, d_result, variant, stream);
+}
+int flsgpu_compute_bp_u64(const flsgpu_bp_column *col, uint32_t n_repetitions,
+                          uint8_t *d_result, int variant, flsgpu_stream_t stream) {
+  FforArgs<uint64_t> a;
+  int rc = make_ffor_args<uint64_t>(col, nullptr, &a);
+  return rc ? rc : launch_compute(a, n_repetitions, d_result, variant, stream);
+}
 int flsgpu_compute_ffor_u32(const flsgpu_ffor_column *col, uint32_t n_repetitions,
                             uint8_t *d_result, int variant, flsgpu_stream_t stream) {
   if (!col || (!col->bases && col->bp.n_vecs)) return set_error(FLSGPU_EINVAL, "null column/bases");
   FforArgs<uint32_t> a;
-  int rc = make_args<uint32_t>(&col->bp, col->bases, &a);
+  int rc = make_ffor_args<uint32_t>(&col->bp, col->bases, &a);
   return rc ? rc : launch_compute(a, n_repetitions, d_result, variant, stream);
 }
 int flsgpu_compute_ffor_u64(const flsgpu_ffor_column *col, uint32_t n_repetitions,
                             uint8_t *d_result, int variant, flsgpu_stream_t stream) {
   if (!col || (!col->bases && col->bp.n_vecs)) return set_error(FLSGPU_EINVAL, "null column/bases");
   FforArgs<uint64_t> a;
-  int rc = make_args<uint64_t>(&col->bp, col->bases, &a);
+  int rc = make_ffor_args<uint64_t>(&col->bp, col->bases, &a);
   return rc ? rc : launch_compute(a, n_repetitions, d_result, variant, stream);
 }
 

@@ -23,6 +23,21 @@ struct MultiFforArgs {
   uint32_t n_cols;
 };
 
+// Checks a BP/FFOR descriptor (bases may be null for plain bit-packing) and
+// converts it to kernel arguments.
+template <typename UT>
+int make_ffor_args(const flsgpu_bp_column *bp, const void *bases, FforArgs<UT> *a) {
+  if (!bp) return set_error(FLSGPU_EINVAL, "null column");
+  if (bp->n_values != bp->n_vecs * VALUES_PER_VECTOR)
+    return set_error(FLSGPU_EINVAL, "n_values (%llu) must equal n_vecs * 1024 (n_vecs = %llu)",
+                     (unsigned long long)bp->n_values, (unsigned long long)bp->n_vecs);
+  if (bp->n_vecs > 0 && (!bp->packed_array || !bp->bit_widths || !bp->vector_offsets))
+    return set_error(FLSGPU_EINVAL, "null device pointer in column");
+  *a = FforArgs<UT>{static_cast<const UT *>(bp->packed_array), bp->bit_widths, bp->vector_offsets,
+                    static_cast<const UT *>(bases), bp->n_vecs};
+  return FLSGPU_OK;
+}
+
 template <typename UT, int S, bool NT>
 struct FlsDecompress {
   static int launch(const FforArgs<UT> &a, UT *out, hipStream_t stream);

@@ -399,7 +399,7 @@ int FlsDecompressStaged<UT, S>::launch(const FforArgs<UT> &a, UT *out, hipStream
 template <typename UT, int S>
 int FlsAny<UT, S>::launch(const FforArgs<UT> &a, UT magic, uint8_t *d_found, hipStream_t stream) {
   Scratch s;
-  int rc = get_scratch(&s);
+  int rc = get_scratch(stream, &s);
   if (rc) return rc;
   const uint64_t n_waves = n_waves_for<UT, S>(a.n_vecs);
   const unsigned grid = std::min(reduce_grid(n_waves), SCAN_GRID);
@@ -413,7 +413,7 @@ template <typename UT, int S, int PRED>
 int FlsFilterSum<UT, S, PRED>::launch(const FforArgs<UT> &a, UT lo, UT hi, uint64_t *d_result,
                                       hipStream_t stream) {
   Scratch s;
-  int rc = get_scratch(&s);
+  int rc = get_scratch(stream, &s);
   if (rc) return rc;
   const uint64_t n_waves = n_waves_for<UT, S>(a.n_vecs);
   const unsigned grid = std::min(reduce_grid(n_waves), SCAN_GRID);
@@ -427,7 +427,7 @@ template <typename UT>
 int FlsCompute<UT>::launch(const FforArgs<UT> &a, uint32_t reps, uint8_t *d_found,
                            hipStream_t stream) {
   Scratch s;
-  int rc = get_scratch(&s);
+  int rc = get_scratch(stream, &s);
   if (rc) return rc;
   const uint64_t n_waves = n_waves_for<UT, 1>(a.n_vecs);
   const unsigned grid = reduce_grid(n_waves);
@@ -441,7 +441,7 @@ template <typename UT>
 int FlsMulti<UT>::launch(const MultiFforArgs<UT> &a, UT value, uint8_t *d_found,
                          hipStream_t stream) {
   Scratch s;
-  int rc = get_scratch(&s);
+  int rc = get_scratch(stream, &s);
   if (rc) return rc;
   const uint64_t n_waves = n_waves_for<UT, 1>(a.col[0].n_vecs);
   const unsigned grid = reduce_grid(n_waves);

@@ -218,3 +218,51 @@ def test_read_file_as_rules(tmp_path):
 def test_query_value_bits():
     assert np.float32(E2E.query_value(np.float32)).view(np.uint32) == 0x4100F9DB
     assert np.float64(E2E.query_value(np.float64)).view(np.uint64) == 0x4100F9DB
+
+
+def test_unsorted_or_duplicate_positions_are_rejected(tmp_path):
+    """The decoders take exception positions as strictly increasing within a
+    vector's list (within a lane's list for G-ALP): the writer, the upload
+    helper's planner and the loader reject any other list."""
+    rng = np.random.default_rng(6)
+    x = decimal_values(rng, 64, 4 * 1024)
+    x[::61] = rng.random(len(x[::61])) * 1e3  # ~17 exceptions per vector
+    alp = AL.alp_encode(x)
+    assert alp is not None and int(alp.counts[1]) >= 3
+    galp = AL.galp_convert(alp)
+    rd = AL.alprd_encode(rng.random(4 * 1024))
+    e1 = int(alp.exceptions_offsets[1])
+
+    # loader: duplicate the second position of vector 1 into the third
+    p = fresh(tmp_path, alp)
+    pos, _, _ = section_pos(p, "positions")
+    patch_bytes(p, pos + 2 * (e1 + 2), alp.positions[e1 + 1].tobytes())
+    rejected(p)
+    # loader: swap two positions of vector 1 (out of order, still unique)
+    p = fresh(tmp_path, alp)
+    pos, _, _ = section_pos(p, "positions")
+    patch_bytes(p, pos + 2 * (e1 + 1), alp.positions[e1 + 2].tobytes() + alp.positions[e1 + 1].tobytes())
+    rejected(p)
+    # loader, G-ALP: a lane's first entry moved to a position of another lane
+    lane = int(np.nonzero(galp.offsets_counts[16:32] >> 10)[0][0])
+    oc = int(galp.offsets_counts[16 + lane])
+    first = int(galp.exceptions_offsets[1]) + (oc & 0x3FF)
+    p = fresh(tmp_path, galp)
+    pos, _, _ = section_pos(p, "positions")
+    patch_bytes(p, pos + 2 * first, np.uint16((int(galp.positions[first]) + 1) % 1024).tobytes())
+    rejected(p)
+    # writer (the same planner runs for flsgpu_column_upload)
+    bad = AL.AlpHostColumn(**{k: (v.copy() if isinstance(v, np.ndarray) else v)
+                              for k, v in vars(alp).items()})
+    bad.positions[e1 + 2] = bad.positions[e1 + 1]
+    with pytest.raises(A.FlsgpuError):
+        CF.write(str(tmp_path / "dup.flsc"), bad)
+    if len(rd.positions) >= 2 and int(rd.counts[0]) >= 2:
+        badrd = AL.AlprdHostColumn(**{k: (v.copy() if isinstance(v, np.ndarray) else v)
+                                      for k, v in vars(rd).items()})
+        badrd.positions[1] = badrd.positions[0]
+        with pytest.raises(A.FlsgpuError):
+            CF.write(str(tmp_path / "duprd.flsc"), badrd)
+    # the undamaged columns still write and read
+    for col in (alp, galp, rd):
+        CF.read_host(fresh(tmp_path, col, "ok.flsc"))
