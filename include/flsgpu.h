@@ -279,11 +279,7 @@ int flsgpu_query_multi_column_alp_f64(const flsgpu_alp_column *cols, uint32_t n_
                                       double value, uint8_t *d_found, int patcher, int variant,
                                       flsgpu_stream_t stream);
 
-/* ---- device-side encode: CURRENTLY DISABLED, every call returns
- *      FLSGPU_EUNSUPPORTED (its first GPU test run faulted the GPU; cause not
- *      yet identified). Interface as documented below.
- *
- *      Device-side encode (no reference counterpart: the reference encodes on
+/* ---- device-side encode (no reference counterpart: the reference encodes on
  *      the host only, src/alp/alp-bindings.cu:83-206, src/fls/ffor.cpp). The
  *      output bytes equal the host encoder's: flsgpu_alp_encode_with_params_*
  *      (encode_simdized + analyze_ffor + FastLanes FFOR, src/alp/encoder.hpp:
@@ -299,7 +295,9 @@ int flsgpu_query_multi_column_alp_f64(const flsgpu_alp_column *cols, uint32_t n_
  *      than 63 exceptions (the 16-bit header cannot encode it; the column is
  *      then unusable in that layout). d_workspace: flsgpu_encode_workspace_bytes
  *      (n_vecs) device bytes. exps / facs: per-vector exponent (<= 10 / 18) and
- *      factor (<= exponent, <= 9 for floats), device arrays. */
+ *      factor (<= exponent, <= 9 for floats), device arrays; a vector whose
+ *      pair is outside these ranges is encoded with (0, 0), which is lossless
+ *      (every value that does not round-trip becomes an exception). */
 size_t flsgpu_encode_workspace_bytes(uint64_t n_vecs);
 int flsgpu_alp_encode_device_f32(const float *in, uint64_t n_vecs, const uint8_t *exps,
                                  const uint8_t *facs, int galp, flsgpu_alp_column *col,

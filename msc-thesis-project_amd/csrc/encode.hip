@@ -177,6 +177,22 @@ __device__ __forceinline__ VecInfo<T, IS_ALP> analyze_vector(const T *__restrict
   return r;
 }
 
+// The (exponent, factor) pair a vector is encoded with: the caller's, when
+// it indexes the tables (exponent <= 10 for floats, <= 18 for doubles;
+// factor <= exponent, and <= 9 for floats, whose factor table has ten
+// entries, src/alp/constants.hpp:49-55), else (0, 0). Any pair decodes
+// losslessly (values that do not round-trip become exceptions), so an
+// out-of-range request costs compression ratio, never correctness or a
+// read outside the tables.
+template <typename T>
+__device__ __forceinline__ void checked_params(unsigned e_in, unsigned f_in, int *exp, int *fac) {
+  constexpr unsigned MAX_E = sizeof(T) == 4 ? 10u : 18u;
+  constexpr unsigned MAX_F = sizeof(T) == 4 ? 9u : 18u;
+  const bool ok = e_in <= MAX_E && f_in <= e_in && f_in <= MAX_F;
+  *exp = ok ? int(e_in) : 0;
+  *fac = ok ? int(f_in) : 0;
+}
+
 struct EncOut {
   void *packed;
   uint8_t *bit_widths;
@@ -207,7 +223,8 @@ __global__ __launch_bounds__(ENC_BLOCK) void k_enc_analyze(const T *__restrict i
   const uint64_t v = uint64_t(blockIdx.x) * VPB + vl;
   uint64_t words = 0, n_exc = 0;
   if (v < n_vecs) {
-    const int fac = IS_ALP ? facs[v] : 0, exp = IS_ALP ? exps[v] : 0;
+    int fac = 0, exp = 0;
+    if constexpr (IS_ALP) checked_params<T>(exps[v], facs[v], &exp, &fac);
     const auto r = analyze_vector<T, IS_ALP>(in + v * VALUES_PER_VECTOR, lane, fac, exp);
     const int bw = bit_count<UT>(UT(UT(r.mx) - UT(r.mn)));
     words = uint64_t(bw) * L::N;
@@ -330,8 +347,7 @@ __global__ __launch_bounds__(ENC_BLOCK) void k_enc_pack(const T *__restrict in, 
   const T *vin = in + v * VALUES_PER_VECTOR;
   if (live) {
     if constexpr (IS_ALP) {
-      fac = facs[v];
-      exp = exps[v];
+      checked_params<T>(exps[v], facs[v], &exp, &fac);
       fill = ST(UT(fills[v]));
       for (int j = 0; j < L::W; ++j) {
         bool exc;
@@ -428,20 +444,10 @@ int encode_launch(const T *in, uint64_t n_vecs, const uint8_t *exps, const uint8
   return check_hip(hipGetLastError(), "encode launch");
 }
 
-// The device encoder is switched off: its first GPU test run left the GPU
-// with a memory fault and the faulting access has not been identified. The
-// entry points reject every call until it is.
-constexpr bool DEVICE_ENCODE_ENABLED = false;
-int encode_disabled() {
-  return set_error(FLSGPU_EUNSUPPORTED,
-                   "device-side encode is disabled (under investigation); use the host encoder");
-}
-
 template <typename T>
 int alp_encode_entry(const T *in, uint64_t n_vecs, const uint8_t *exps, const uint8_t *facs,
                      int galp, flsgpu_alp_column *col, uint64_t *d_totals, void *workspace,
                      hipStream_t stream) {
-  if (!DEVICE_ENCODE_ENABLED) return encode_disabled();
   if (!col) return set_error(FLSGPU_EINVAL, "null column");
   EncOut o{const_cast<void *>(col->ffor.bp.packed_array),
            const_cast<uint8_t *>(col->ffor.bp.bit_widths),
@@ -467,7 +473,6 @@ int alp_encode_entry(const T *in, uint64_t n_vecs, const uint8_t *exps, const ui
 template <typename UT>
 int ffor_encode_entry(const UT *in, uint64_t n_vecs, flsgpu_ffor_column *col, uint64_t *d_totals,
                       void *workspace, hipStream_t stream) {
-  if (!DEVICE_ENCODE_ENABLED) return encode_disabled();
   if (!col) return set_error(FLSGPU_EINVAL, "null column");
   EncOut o{const_cast<void *>(col->bp.packed_array),
            const_cast<uint8_t *>(col->bp.bit_widths),

@@ -132,6 +132,14 @@ _SIGS = {
     "flsgpu_query_any_alp_f64": ([C.POINTER(ALPColumn), C.c_double, _P, C.c_int, C.c_int, _P], C.c_int),
     "flsgpu_query_any_alp_ext_f32": ([C.POINTER(ALPColumn), C.c_float, _P, C.c_int, C.c_int, _P], C.c_int),
     "flsgpu_query_any_alp_ext_f64": ([C.POINTER(ALPColumn), C.c_double, _P, C.c_int, C.c_int, _P], C.c_int),
+    "flsgpu_synchronize": ([_P], C.c_int),
+    "flsgpu_release_stream": ([_P], C.c_int),
+    "flsgpu_query_filter_sum_bp_u32": (
+        [C.POINTER(BPColumn), C.c_int, C.c_uint64, C.c_uint64, _P, C.c_int, _P], C.c_int),
+    "flsgpu_query_filter_sum_bp_u64": (
+        [C.POINTER(BPColumn), C.c_int, C.c_uint64, C.c_uint64, _P, C.c_int, _P], C.c_int),
+    "flsgpu_compute_bp_u32": ([C.POINTER(BPColumn), C.c_uint32, _P, C.c_int, _P], C.c_int),
+    "flsgpu_compute_bp_u64": ([C.POINTER(BPColumn), C.c_uint32, _P, C.c_int, _P], C.c_int),
     "flsgpu_query_filter_sum_ffor_u32": (
         [C.POINTER(FFORColumn), C.c_int, C.c_uint64, C.c_uint64, _P, C.c_int, _P], C.c_int),
     "flsgpu_query_filter_sum_ffor_u64": (

@@ -125,11 +125,10 @@ def query_filter_sum(col: DeviceFFORColumn, predicate: int = A.PRED_OFFSET_LOWER
     if result is None:
         result = torch.empty(2, dtype=torch.uint64, device=col.packed.device)
     L = A.lib()
-    fn = L.flsgpu_query_filter_sum_ffor_u32 if col.width == 32 else L.flsgpu_query_filter_sum_ffor_u64
-    bases = col.bases
-    if bases is None:
-        raise ValueError("filter_sum needs an FFOR column (bases)")
-    A.check(fn(C.byref(col.ffor_struct()), predicate, int(lo), int(hi), _ptr(result), variant,
+    kind = "bp" if col.bases is None else "ffor"
+    fn = getattr(L, f"flsgpu_query_filter_sum_{kind}_u{col.width}")
+    desc = col.bp_struct() if col.bases is None else col.ffor_struct()
+    A.check(fn(C.byref(desc), predicate, int(lo), int(hi), _ptr(result), variant,
                _stream_ptr(stream)))
     return result
 
@@ -140,9 +139,10 @@ def compute(col: DeviceFFORColumn, n_repetitions: int = 10, result: torch.Tensor
     if result is None:
         result = torch.empty(1, dtype=torch.uint8, device=col.packed.device)
     L = A.lib()
-    fn = L.flsgpu_compute_ffor_u32 if col.width == 32 else L.flsgpu_compute_ffor_u64
-    A.check(fn(C.byref(col.ffor_struct()), n_repetitions, _ptr(result), variant,
-               _stream_ptr(stream)))
+    kind = "bp" if col.bases is None else "ffor"
+    fn = getattr(L, f"flsgpu_compute_{kind}_u{col.width}")
+    desc = col.bp_struct() if col.bases is None else col.ffor_struct()
+    A.check(fn(C.byref(desc), n_repetitions, _ptr(result), variant, _stream_ptr(stream)))
     return result
 
 

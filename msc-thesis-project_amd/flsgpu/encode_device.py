@@ -17,7 +17,7 @@ _UT = {32: torch.uint32, 64: torch.uint64}
 
 def _sigs():
     L = A.lib()
-    if not getattr(L, "_enc_sigs", False):
+    if not getattr(L, "_device_encode_sigs", False):
         L.flsgpu_encode_workspace_bytes.argtypes = [C.c_uint64]
         L.flsgpu_encode_workspace_bytes.restype = C.c_size_t
         for t in ("f32", "f64"):
@@ -28,7 +28,7 @@ def _sigs():
             f = getattr(L, f"flsgpu_ffor_encode_device_{t}")
             f.argtypes = [_P, C.c_uint64, C.POINTER(A.FFORColumn), _P, _P, _P]
             f.restype = C.c_int
-        L._enc_sigs = True
+        L._device_encode_sigs = True
     return L
 
 

@@ -123,3 +123,32 @@ def test_batch_plan_rejects_null_alp_arrays(field):
         rc, h = _plan(enc, col, patcher)
         if needs:
             assert rc == A.FLSGPU_EINVAL and not h.value, (enc, field)
+
+
+def test_every_declared_entry_point_has_a_ctypes_signature():
+    """Without argtypes ctypes passes a Python int as a 32-bit C int, which
+    truncates 64-bit device pointers and sizes; every function the headers
+    declare with parameters must have its signature set by some wrapper."""
+    import re
+
+    from flsgpu import alp as AL
+    from flsgpu import batch as B
+    from flsgpu import colfile as CF
+    from flsgpu import encode_device as ED
+    from flsgpu import selector as SEL
+    L = A.lib()
+    AL._enc_lib()
+    B._sigs()
+    CF._lib()
+    ED._sigs()
+    SEL._lib()
+    no_params = set()
+    import os
+    inc = os.path.dirname(A.HEADER_PATH)
+    for f in sorted(os.listdir(inc)):
+        if f.endswith(".h"):
+            text = re.sub(r"/\*.*?\*/", "", open(os.path.join(inc, f)).read(), flags=re.S)
+            no_params.update(re.findall(r"\b(flsgpu_[a-z0-9_]+)\s*\(\s*void\s*\)", text))
+    missing = [n for n in A.header_symbols() if n not in no_params
+               and getattr(getattr(L, n), "argtypes", None) is None]
+    assert not missing, f"no ctypes signature: {missing}"

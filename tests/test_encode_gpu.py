@@ -6,11 +6,7 @@ import numpy as np
 import pytest
 
 torch = pytest.importorskip("torch")
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="device-side encode is switched off: its first GPU run faulted the "
-                            "GPU and the faulting access is not yet identified"),
-]
+pytestmark = pytest.mark.gpu
 
 from flsgpu import _capi as A  # noqa: E402
 from flsgpu import alp as AL  # noqa: E402
@@ -103,6 +99,32 @@ def test_alp_device_encode_matches_host(dtype, mode, galp):
         assert np.array_equal(t2n(col.offsets_counts), host.offsets_counts)
     out = AL.decompress(col).cpu().numpy()
     assert np.array_equal(bits(out), bits(x))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_out_of_range_parameters_encode_as_zero_zero(dtype):
+    """(exponent, factor) pairs outside the tables (exponent above 10 / 18,
+    factor above the exponent, float factor 10) encode their vector with
+    (0, 0): the same bytes as the host encoder at (0, 0), lossless."""
+    rng = np.random.default_rng(8)
+    x = np.round(rng.uniform(-100, 100, 4 * 1024), 1).astype(dtype)
+    x[::7] = np.round(x[::7])  # some integers, which (0, 0) encodes without exceptions
+    maxe = 10 if dtype == np.float32 else 18
+    e = np.array([maxe + 1, 3, 255, 10], np.uint8)
+    f = np.array([0, 4, 0, 10], np.uint8)   # factor > exponent; factor 10 (floats)
+    ok = np.array([False, False, False, dtype == np.float64])
+    e0 = np.where(ok, e, 0).astype(np.uint8)
+    f0 = np.where(ok, f, 0).astype(np.uint8)
+    host = AL.alp_encode_with_params(x, e0, f0)
+    xd = torch.from_numpy(x).to(DEV)
+    col, info = ED.alp_encode_device(xd, torch.from_numpy(e).to(DEV), torch.from_numpy(f).to(DEV))
+    assert np.array_equal(t2n(col.factor_indices), f0)
+    assert np.array_equal(t2n(col.fraction_indices), e0)
+    nw, ne = info["packed_words"], info["exceptions"]
+    assert nw == len(host.packed) and ne == len(host.exceptions)
+    assert np.array_equal(t2n(col.packed, nw), host.packed)
+    assert np.array_equal(bits(t2n(col.exceptions, ne)), bits(host.exceptions))
+    assert np.array_equal(bits(AL.decompress(col).cpu().numpy()), bits(x))
 
 
 def test_galp_lane_overflow_is_reported():
