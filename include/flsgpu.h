@@ -248,6 +248,19 @@ int flsgpu_query_filter_sum_ffor_u64(const flsgpu_ffor_column *col, int predicat
                                      uint64_t hi, uint64_t *d_result, int variant,
                                      flsgpu_stream_t stream);
 
+/* The same fused filter + sum for 32-bit FFOR columns, allocating nothing.
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the block partials live in a static 32 KiB device array
+ * of the library (2048 partial pairs; the grid is capped at that, so any
+ * column size works), one copy per device, set up by flsgpu_init or the
+ * library's first call on the device. Every call on a device shares that
+ * device's array: calls that overlap on one device, on any streams, race;
+ * calls ordered on one stream do not. */
+int flsgpu_query_filter_sum_ffor_u32_noalloc(const flsgpu_ffor_column *col, int predicate,
+                                             uint64_t lo, uint64_t hi, uint64_t *d_result,
+                                             flsgpu_stream_t stream);
+
 /* ---- synthetic ALU load: kernels::host::compute_column (kernels.cuh:135-153,
  *      device :56-89): per value n_repetitions of *=3, <<=3, +=3, >>=3, &=0,
  *      then "any == 1"; *d_result is written with 0 or 1. */

@@ -33,7 +33,6 @@ int check_hip(hipError_t e, const char *what) {
   return set_error(FLSGPU_EHIP, "%s: %s", what, hipGetErrorString(e));
 }
 
-constexpr int MAX_DEVICES = 64;
 // Reduction areas per device, one per stream that ran a reduction. A stream
 // handle is only a key here: the area of a destroyed stream stays until
 // flsgpu_release_stream, and a new stream that reuses the handle reuses the
@@ -62,8 +61,16 @@ static int init_device_locked(int dev) {
   int rc = check_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
   if (rc) return rc;
   g_cus[dev] = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  if ((rc = setup_static_scratch(dev))) return rc;
   g_ready[dev] = true;
   return FLSGPU_OK;
+}
+
+int ready_device(int *dev) {
+  int rc = current_device(dev);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_mu);
+  return init_device_locked(*dev);
 }
 
 int get_scratch(hipStream_t stream, Scratch *s) {

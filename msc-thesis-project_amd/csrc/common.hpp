@@ -127,6 +127,8 @@ struct Scratch {
   uint64_t *partials;  // [MAX_REDUCE_BLOCKS * 2]
 };
 
+constexpr int MAX_DEVICES = 64;
+
 // Status helpers (capi.hip)
 int set_error(int code, const char *fmt, ...);
 int check_hip(hipError_t e, const char *what);
@@ -134,6 +136,12 @@ int check_hip(hipError_t e, const char *what);
 // stream's first reduction, kept until flsgpu_release_stream or exit).
 int get_scratch(hipStream_t stream, Scratch *s);
 int num_cus();
+// Initialises the current device on first use (properties, the static
+// scratch address below) and returns its index.
+int ready_device(int *dev);
+// Looks up this device's copy of the static scratch array (noalloc.hip);
+// called once per device, with that device current, by the initialisation.
+int setup_static_scratch(int dev);
 // Grid for grid-stride reduction kernels: one wave per unit of work, at
 // most MAX_REDUCE_BLOCKS blocks.
 unsigned reduce_grid(uint64_t n_waves);

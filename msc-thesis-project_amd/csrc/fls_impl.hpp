@@ -54,6 +54,10 @@ struct FlsAny {
 template <typename UT, int S, int PRED>
 struct FlsFilterSum {
   static int launch(const FforArgs<UT> &a, UT lo, UT hi, uint64_t *d_result, hipStream_t stream);
+  // The same two launches over partials the caller provides: room for
+  // `slots` blocks (the grid is capped at that); allocates nothing.
+  static int launch_partials(const FforArgs<UT> &a, UT lo, UT hi, uint64_t *d_result,
+                             uint64_t *partials, unsigned slots, hipStream_t stream);
 };
 template <typename UT>
 struct FlsCompute {

@@ -415,11 +415,21 @@ int FlsFilterSum<UT, S, PRED>::launch(const FforArgs<UT> &a, UT lo, UT hi, uint6
   Scratch s;
   int rc = get_scratch(stream, &s);
   if (rc) return rc;
+  return launch_partials(a, lo, hi, d_result, s.partials, MAX_REDUCE_BLOCKS, stream);
+}
+
+template <typename UT, int S, int PRED>
+int FlsFilterSum<UT, S, PRED>::launch_partials(const FforArgs<UT> &a, UT lo, UT hi,
+                                               uint64_t *d_result, uint64_t *partials,
+                                               unsigned slots, hipStream_t stream) {
   const uint64_t n_waves = n_waves_for<UT, S>(a.n_vecs);
-  const unsigned grid = std::min(reduce_grid(n_waves), SCAN_GRID);
+  const unsigned grid = std::min({reduce_grid(n_waves), SCAN_GRID, slots});
+  if (grid == 0) return set_error(FLSGPU_EINVAL, "no room for block partials");
+  const Scratch s{partials};
   hipLaunchKernelGGL((k_ffor_filter_sum<UT, S, PRED>), dim3(grid), dim3(BLOCK), 0, stream, a, lo,
                      hi, n_waves, s);
-  if ((rc = check_hip(hipGetLastError(), "k_ffor_filter_sum launch"))) return rc;
+  int rc = check_hip(hipGetLastError(), "k_ffor_filter_sum launch");
+  if (rc) return rc;
   return reduce_finish_pair(s, grid, d_result, stream);
 }
 

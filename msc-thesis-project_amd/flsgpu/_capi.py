@@ -144,6 +144,8 @@ _SIGS = {
         [C.POINTER(FFORColumn), C.c_int, C.c_uint64, C.c_uint64, _P, C.c_int, _P], C.c_int),
     "flsgpu_query_filter_sum_ffor_u64": (
         [C.POINTER(FFORColumn), C.c_int, C.c_uint64, C.c_uint64, _P, C.c_int, _P], C.c_int),
+    "flsgpu_query_filter_sum_ffor_u32_noalloc": (
+        [C.POINTER(FFORColumn), C.c_int, C.c_uint64, C.c_uint64, _P, _P], C.c_int),
     "flsgpu_compute_ffor_u32": ([C.POINTER(FFORColumn), C.c_uint32, _P, C.c_int, _P], C.c_int),
     "flsgpu_compute_ffor_u64": ([C.POINTER(FFORColumn), C.c_uint32, _P, C.c_int, _P], C.c_int),
     "flsgpu_query_multi_column_ffor_u32": (
