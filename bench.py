@@ -177,12 +177,18 @@ def main():
         "per_width": per_width,
     }
 
+    host = {}  # host copies of the GPU run's buffers for the CPU leg
+    if not args.no_cpu:
+        host["sweep"] = [(w, bw, host_prefix(c, 1 << 10)) for w, bw, c in cols]
+        host["c2"] = {(w, bw): host_prefix(c, 1 << 14) for w, bw, c in cols if (w, bw) in C2_CPU}
     del cols
     torch.cuda.empty_cache()
     if not args.no_extras:
-        result["extras"] = extras(args, torch, A, D, E, dev, stream, out32)
+        result["extras"] = extras(args, torch, A, D, E, dev, stream, out32,
+                                  None if args.no_cpu else host)
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(args)
+        print("[bench] CPU baseline", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline(args, host)
     line = json.dumps(result)
     print(line, flush=True)
     if args.out:
@@ -294,7 +300,7 @@ def time_launches(torch, stream, fn, reps):
     return a.elapsed_time(b) / reps
 
 
-def extras(args, torch, A, D, E, dev, stream, out32):
+def extras(args, torch, A, D, E, dev, stream, out32, host=None):
     """Other configurations (not the headline): C1 u32 w=13 column, fused
     filter+sum sweep at 2^28 values."""
     import numpy as np
@@ -303,12 +309,12 @@ def extras(args, torch, A, D, E, dev, stream, out32):
     r = torch.empty(2, dtype=torch.uint64, device=dev)
     flush = CacheFlush(torch, dev)
     if "c1" in which:
-        res.update(c1_extras(args, torch, A, D, E, dev, stream, r, flush))
+        res.update(c1_extras(args, torch, A, D, E, dev, stream, r, flush, host))
     if "filter" in which:
         res["filter_sum_sweep"] = filter_sum_extras(args, torch, A, D, E, dev, stream, r, flush)
     torch.cuda.empty_cache()
     if "alp" in which:
-        res.update(alp_extras(torch, dev, stream, flush))
+        res.update(alp_extras(torch, dev, stream, flush, host))
     del flush
     if "batch" in which:
         res["c4_batch"] = batch_extras(torch, dev, stream)
@@ -317,7 +323,7 @@ def extras(args, torch, A, D, E, dev, stream, out32):
     return res
 
 
-def c1_extras(args, torch, A, D, E, dev, stream, r, flush):
+def c1_extras(args, torch, A, D, E, dev, stream, r, flush, host=None):
     """2^25 uint32 values 1e6 + U[0, 2^13), FOR base = vector min, width 13:
     decode to HBM and fused sum; every timed launch starts with cold caches."""
     import numpy as np
@@ -326,6 +332,10 @@ def c1_extras(args, torch, A, D, E, dev, stream, r, flush):
     vals = (1_000_000 + rng.integers(0, 1 << 13, 1 << 25)).astype(np.uint32)
     p, b, o, s = E.ffor_encode_uniform(vals, 13)
     col = D.DeviceFFORColumn.from_host(p, b, o, s, len(vals), dev)
+    if host is not None:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        host["c1"] = O.FFORColumn(len(vals), p, b.astype(np.uint8), o.astype(np.uint64), s)
     out = torch.empty(len(vals), dtype=torch.uint32, device=dev)
     ms = cold_launch_ms(torch, stream, lambda: D.decompress(col, out, variant=args.variant,
                                                             stream=stream), 20, flush)
@@ -458,7 +468,7 @@ def c3_values(np, width, rng):
     return x
 
 
-def alp_extras(torch, dev, stream, flush):
+def alp_extras(torch, dev, stream, flush, host=None):
     import numpy as np
     from flsgpu import alp as AL
     res = {}
@@ -469,6 +479,10 @@ def alp_extras(torch, dev, stream, flush):
         col = AL.alp_encode(x)
         enc_s = time.perf_counter() - t0
         galp = AL.galp_convert(col)
+        if host is not None:
+            name = f"c3_alp_f{width}_2^{int(np.log2(len(x)))}"
+            host.setdefault("c3", {})[name] = col
+            host.setdefault("c3_values", {})[name] = x
         ref = torch.from_numpy(x.view(np.int64 if width == 64 else np.int32)).to(dev)
         nbytes = x.nbytes
         for name, c in (("alp", col), ("galp", galp)):
@@ -522,65 +536,139 @@ def alp_extras(torch, dev, stream, flush):
     return res
 
 
-def cpu_baseline(args):
-    """The reference's own CPU decoder (fls::unffor per vector, the loop of
-    src/engine/data.cuh:300-313), compiled from its sources into oracle/_ref,
-    on a bounded sample of the same sweep: 2^20 values per (dtype, width)."""
+def host_prefix(col, n_vecs):
+    """The first n_vecs vectors of a device FFOR column, copied to host as an
+    oracle column (the CPU leg decodes the GPU run's own buffers)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+    n_vecs = min(n_vecs, col.n_vecs)
+    nl = 1024 // col.width
+    bws = col.bit_widths[:n_vecs].cpu().numpy()
+    offs = col.vector_offsets[:n_vecs].cpu().numpy()
+    end = int(offs[-1]) + int(bws[-1]) * nl
+    packed = col.packed[:max(end, 1)].cpu().numpy()
+    bases = col.bases[:n_vecs].cpu().numpy()
+    return O.FFORColumn(n_vecs * 1024, packed, bws, offs, bases)
+
+
+# C2 columns the CPU leg also times on their own: (element bits, bit width)
+C2_CPU = {(32, 13), (32, 16), (32, 32), (64, 13), (64, 16), (64, 32), (64, 64)}
+CPU_RUNS = 11  # timed runs per figure after one warm-up; the median is reported
+
+
+def _median_s(fn, runs=CPU_RUNS):
+    import statistics
+    fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return statistics.median(ts)
+
+
+def _ranges(n, k):
+    return [(n * i // k, n * (i + 1) // k) for i in range(k) if n * i // k < n * (i + 1) // k]
+
+
+def cpu_baseline(args, host):
+    """The reference's own CPU decoders, compiled from its sources into
+    oracle/_ref, on the GPU run's buffers, one thread and all host threads (at
+    most 16, the CPU share of a one-GPU job; the ctypes calls release the GIL,
+    so a thread pool over contiguous vector ranges runs them in parallel).
+    Each figure is the median of CPU_RUNS runs after a warm-up.
+
+    - headline: the bit-width sweep (u32 w=0..32, u64 w=0..64), the first
+      2^20 values of every device column: fls::unffor per vector, the loop of
+      src/engine/data.cuh:300-313;
+    - C2 widths 13/16/32 (u32) and 13/16/32/64 (u64): the first 2^24 values
+      of those device columns;
+    - C1: the whole 2^25-value width-13 column;
+    - C3: the whole ALP columns (2^26 doubles, 2^27 floats), standard layout:
+      falp + patch_exceptions per vector (src/alp/alp-bindings.cu:208-228)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle_py as O
-    use_ref = O.ref() is not None
-    kind = "reference" if use_ref else "port"
-    rng = np.random.default_rng(5)
-    n_vecs = 1024
-    samples = []
-    for width in (32, 64):
-        ut = np.uint32 if width == 32 else np.uint64
-        for bw in range(width + 1):
-            nl = 1024 // width
-            packed = np.frombuffer(rng.bytes(n_vecs * bw * nl * width // 8 or 8), dtype=ut).copy()
-            col = O.FFORColumn(n_vecs * 1024, packed, np.full(n_vecs, bw, np.uint8),
-                               np.arange(n_vecs, dtype=np.uint64) * np.uint64(bw * nl),
-                               np.frombuffer(rng.bytes(n_vecs * width // 8), dtype=ut).copy())
-            samples.append(col)
-    dec_bytes = sum(c.n_values * c.packed.itemsize for c in samples)
-    passes = 0
-    t0 = time.perf_counter()
-    while True:
-        for c in samples:
-            O.ffor_column_decode(c, use_ref=use_ref)
-        passes += 1
-        if time.perf_counter() - t0 > 10.0:
-            break
-    dt = time.perf_counter() - t0
-    # All cores: the same columns spread over a thread pool (the ctypes call
-    # releases the GIL, so the decodes run in parallel), at most 16 threads:
-    # the CPU share a one-GPU job gets.
     from concurrent.futures import ThreadPoolExecutor
+    R = O.ref()
+    if R is None:
+        return {"value": None, "unit": "GB/s", "cores": 1, "kind": "reference",
+                "sample": "oracle/_ref is not built: no CPU baseline"}
     n_threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    mt_passes = 0
-    with ThreadPoolExecutor(n_threads) as pool:
-        t1 = time.perf_counter()
-        while True:
-            list(pool.map(lambda c: O.ffor_column_decode(c, use_ref=use_ref), samples))
-            mt_passes += 1
-            if time.perf_counter() - t1 > 5.0:
-                break
-        mt_dt = time.perf_counter() - t1
-    model = ""
+    pool = ThreadPoolExecutor(n_threads)
+
+    def ffor_fn(col, out):
+        isz = col.packed.itemsize
+        f = getattr(R, f"ref_ffor_column_decode_u{isz * 8}")
+        pk, bw = col.packed.ctypes.data, col.bit_widths.ctypes.data
+        of, bs, od = col.vector_offsets.ctypes.data, col.bases.ctypes.data, out.ctypes.data
+
+        def run(v0, v1, keep=(col, out)):  # keep: the arrays the addresses point into
+            f(v1 - v0, pk, bw + v0, of + 8 * v0, bs + isz * v0, od + 1024 * isz * v0)
+        return run
+
+    def alp_fn(col, out):
+        return lambda v0, v1: O.ref_alp_column_decode_range(col, v0, v1, out)
+
+    def timed(items):
+        """items: [(decode(v0, v1), n_vecs, decompressed bytes)] decoded in
+        one run; returns (GB/s on 1 thread, GB/s on n_threads)."""
+        nbytes = sum(b for _, _, b in items)
+        t1 = _median_s(lambda: [fn(0, n) for fn, n, _ in items])
+
+        def par():
+            futs = [pool.submit(fn, a, b) for fn, n, _ in items for a, b in _ranges(n, n_threads)]
+            for f in futs:
+                f.result()
+        tn = _median_s(par)
+        return round(nbytes / t1 / 1e9, 3), round(nbytes / tn / 1e9, 3)
+
+    def ffor_items(cols):
+        items = []
+        for c in cols:
+            out = np.empty(c.n_vecs * 1024, dtype=c.packed.dtype)
+            items.append((ffor_fn(c, out), c.n_vecs, out.nbytes))
+        return items
+
+    t0 = time.perf_counter()
+    sweep = host["sweep"]
+    one, many = timed(ffor_items([c for _, _, c in sweep]))
+    configs = {}
+    for (width, bw), c in sorted(host.get("c2", {}).items()):
+        a, b = timed(ffor_items([c]))
+        configs[f"c2_u{width}_w{bw}_2^{int(np.log2(c.n_values))}"] = {
+            "threads_1": a, f"threads_{n_threads}": b}
+    if "c1" in host:
+        a, b = timed(ffor_items([host["c1"]]))
+        configs["c1_u32_w13_2^25"] = {"threads_1": a, f"threads_{n_threads}": b}
+    for name, col in host.get("c3", {}).items():
+        out = np.empty(col.n_values, dtype=col.exceptions.dtype)
+        a, b = timed([(alp_fn(col, out), col.bit_widths.size, out.nbytes)])
+        ok = bool(np.array_equal(out.view(np.uint8), host["c3_values"][name].view(np.uint8)))
+        configs[name] = {"threads_1": a, f"threads_{n_threads}": b, "bit_exact": ok}
+    pool.shutdown()
+    info = {}
     try:
         with open("/proc/cpuinfo") as f:
-            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+            for line in f:
+                k, _, v = line.partition(":")
+                k = k.strip()
+                if k in ("model name", "cpu family", "model", "stepping", "cpu MHz") and k not in info:
+                    info[k] = v.strip()
     except OSError:
         pass
-    return {"value": round(dec_bytes * passes / dt / 1e9, 3), "unit": "GB/s", "cores": 1,
-            "kind": kind,
-            "sample": f"same sweep (u32 w=0..32, u64 w=0..64) at 2^20 values per width, "
-                      f"{passes} passes, single thread, fls::unffor per vector",
-            "all_cores": {"value": round(dec_bytes * mt_passes / mt_dt / 1e9, 3), "unit": "GB/s",
-                          "cores": n_threads,
-                          "sample": f"the same columns over {n_threads} threads, {mt_passes} passes"},
-            "cpu_model": model}
+    model = (f"{info.get('model name', '?')} (family {info.get('cpu family', '?')}, model "
+             f"{info.get('model', '?')}, stepping {info.get('stepping', '?')}, "
+             f"{info.get('cpu MHz', '?')} MHz)")
+    return {"value": one, "unit": "GB/s", "cores": 1, "kind": "reference",
+            "sample": f"the first 2^20 values of every sweep column (u32 w=0..32, u64 w=0..64) "
+                      f"copied from the device, fls::unffor per vector, median of {CPU_RUNS} "
+                      f"runs, single thread",
+            "all_cores": {"value": many, "unit": "GB/s", "cores": n_threads,
+                          "sample": f"the same columns, each split over {n_threads} threads"},
+            "configs": configs, "cpu_seconds": round(time.perf_counter() - t0, 1),
+            "cpu_model": model, "host_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0))}
 
 
 if __name__ == "__main__":

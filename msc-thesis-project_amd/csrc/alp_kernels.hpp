@@ -34,24 +34,44 @@ __device__ __forceinline__ T alp_value(typename uint_of<T>::type v, typename uin
   return T(s) * frac;
 }
 
+// The per-vector constants of the value formula. The FACT / FRAC table
+// entries are read in the decode's after-loads hook (see unpack_slots): their
+// indices arrive with the vector's other metadata, and reading the tables
+// before the packed words are requested would add a round trip ahead of them.
+template <typename T>
+struct AlpConsts {
+  using UT = typename uint_of<T>::type;
+  UT base;
+  unsigned fi, ei;
+  UT fact;
+  T frac;
+  __device__ __forceinline__ void resolve() {
+    fact = alp_factor<T>(fi);
+    frac = alp_fraction<T>(ei);
+  }
+  __device__ __forceinline__ T operator()(UT v) const { return alp_value<T>(v, base, fact, frac); }
+};
+
+template <typename T>
+__device__ __forceinline__ AlpConsts<T> alp_consts(const AlpArgs<T> &a, uint64_t vec) {
+  return AlpConsts<T>{a.ffor.bases[vec], a.factor_indices[vec], a.fraction_indices[vec], 0, T(0)};
+}
+
 template <typename T, bool NT>
 struct AlpDecompressPolicy {
   using UT = typename uint_of<T>::type;
   const UT *in;
   T *out;
-  UT base;
-  UT fact;
-  T frac;
+  AlpConsts<T> c;
   template <int BW, int J0, int NJ>
   __device__ __forceinline__ void run() {
-    unpack_slots<UT, BW, J0, NJ>(in, [&](int j, UT v) {
-      store<T, NT>(out + j * lanes<UT>::N, alp_value<T>(v, base, fact, frac));
-    });
+    unpack_slots<UT, BW, J0, NJ>(
+        in, [&](int j, UT v) { store<T, NT>(out + j * lanes<UT>::N, c(v)); }, [&] { c.resolve(); });
   }
   __device__ __forceinline__ void run_generic(int bw, int j0, int nj) {
-    unpack_runtime<UT>(in, bw, j0, nj, [&](int j, UT v) {
-      store<T, NT>(out + j * lanes<UT>::N, alp_value<T>(v, base, fact, frac));
-    });
+    c.resolve();
+    unpack_runtime<UT>(in, bw, j0, nj,
+                       [&](int j, UT v) { store<T, NT>(out + j * lanes<UT>::N, c(v)); });
   }
 };
 
@@ -71,9 +91,7 @@ struct AlpRegPatchPolicy {
   static constexpr int RK = alp_reg_k<T>();
   const UT *in;
   T *out;
-  UT base;
-  UT fact;
-  T frac;
+  AlpConsts<T> c;
   uint64_t emask;
   T q[RK];
   __device__ __forceinline__ T take(int j, T y) {
@@ -85,14 +103,85 @@ struct AlpRegPatchPolicy {
   }
   template <int BW, int J0, int NJ>
   __device__ __forceinline__ void run() {
-    unpack_slots<UT, BW, J0, NJ>(in, [&](int j, UT v) {
-      store<T, NT>(out + j * lanes<UT>::N, take(j, alp_value<T>(v, base, fact, frac)));
-    });
+    unpack_slots<UT, BW, J0, NJ>(
+        in, [&](int j, UT v) { store<T, NT>(out + j * lanes<UT>::N, take(j, c(v))); },
+        [&] { c.resolve(); });
   }
   __device__ __forceinline__ void run_generic(int bw, int j0, int nj) {
-    unpack_runtime<UT>(in, bw, j0, nj, [&](int j, UT v) {
-      store<T, NT>(out + j * lanes<UT>::N, take(j, alp_value<T>(v, base, fact, frac)));
-    });
+    c.resolve();
+    unpack_runtime<UT>(in, bw, j0, nj,
+                       [&](int j, UT v) { store<T, NT>(out + j * lanes<UT>::N, take(j, c(v))); });
+  }
+};
+
+// G-ALP register patching. The lane's first RK list entries are loaded
+// before the decode (load_entries, no branch on the lane's count, so the
+// wait placed before their first use does not also wait for the packed
+// words issued after them) and sorted into the queue once the packed words
+// are in flight (prepare, in the after-loads hook): entries in this thread's
+// slot range go to the queue by slot rank (a repeated slot keeps the later
+// entry). `queued` marks the entries queued.
+template <typename T, bool NT>
+struct AlpGalpRegPolicy : AlpRegPatchPolicy<T, NT> {
+  using UT = typename uint_of<T>::type;
+  using Base = AlpRegPatchPolicy<T, NT>;
+  static constexpr int RK = Base::RK;
+  uint64_t first;
+  unsigned count;
+  int lane, j0, nj;
+  uint64_t queued;
+  unsigned raw[RK];
+  T val[RK];
+  __device__ __forceinline__ void load_entries(const AlpArgs<T> &a) {
+#pragma unroll
+    for (int k = 0; k < RK; ++k) {
+      raw[k] = *list_entry(a.positions, first, unsigned(k), count);
+      val[k] = *list_entry(a.exceptions, first, unsigned(k), count);
+    }
+  }
+  __device__ __forceinline__ void prepare() {
+    constexpr int NL = lanes<UT>::N;
+    unsigned slot[RK];
+#pragma unroll
+    for (int k = 0; k < RK; ++k) {
+      const unsigned sl = raw[k] / unsigned(NL);
+      slot[k] = unsigned(k) < count && int(raw[k] % unsigned(NL)) == lane && sl >= unsigned(j0) &&
+                        sl < unsigned(j0 + nj)
+                    ? sl
+                    : 0xFFFFu;  // not queued
+    }
+    this->emask = 0;
+    queued = 0;
+#pragma unroll
+    for (int k = 0; k < RK; ++k) {
+      this->q[k] = T(0);
+      if (slot[k] != 0xFFFFu) {
+        this->emask |= uint64_t(1) << slot[k];
+        queued |= uint64_t(1) << k;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < RK; ++k) {
+      if (slot[k] != 0xFFFFu) {
+        const int r = __builtin_popcountll(this->emask & ((uint64_t(1) << slot[k]) - 1));
+#pragma unroll
+        for (int i = 0; i < RK; ++i) this->q[i] = r == i ? val[k] : this->q[i];
+      }
+    }
+  }
+  template <int BW, int J0, int NJ>
+  __device__ __forceinline__ void run() {
+    unpack_slots<UT, BW, J0, NJ>(
+        this->in,
+        [&](int j, UT v) { store<T, NT>(this->out + j * lanes<UT>::N, this->take(j, this->c(v))); },
+        [&] {
+          this->c.resolve();
+          prepare();
+        });
+  }
+  __device__ __forceinline__ void run_generic(int bw, int j0_, int nj_) {
+    prepare();
+    Base::run_generic(bw, j0_, nj_);
   }
 };
 
@@ -101,21 +190,19 @@ template <typename T>
 struct AlpMatchPolicy {
   using UT = typename uint_of<T>::type;
   const UT *in;
-  UT base;
-  UT fact;
-  T frac;
+  AlpConsts<T> c;
   T target;
   uint64_t mask;
   template <int BW, int J0, int NJ>
   __device__ __forceinline__ void run() {
-    unpack_slots<UT, BW, J0, NJ>(in, [&](int j, UT v) {
-      mask |= uint64_t(alp_value<T>(v, base, fact, frac) == target) << (j - J0);
-    });
+    unpack_slots<UT, BW, J0, NJ>(
+        in, [&](int j, UT v) { mask |= uint64_t(c(v) == target) << (j - J0); },
+        [&] { c.resolve(); });
   }
   __device__ __forceinline__ void run_generic(int bw, int j0, int nj) {
-    unpack_runtime<UT>(in, bw, j0, nj, [&](int j, UT v) {
-      mask |= uint64_t(alp_value<T>(v, base, fact, frac) == target) << (j - j0);
-    });
+    c.resolve();
+    unpack_runtime<UT>(in, bw, j0, nj,
+                       [&](int j, UT v) { mask |= uint64_t(c(v) == target) << (j - j0); });
   }
 };
 
@@ -172,8 +259,67 @@ __device__ __forceinline__ void galp_own_exceptions(const AlpArgs<T> &a, uint64_
   }
 }
 
+// The first PF * 64 entries of the exception lists of the wave's VPW vectors
+// (entry t + 64 p to thread t of the wave). The per-vector headers are loaded
+// first (wave-uniform), then every list entry, and nothing loaded is used
+// before the last load is issued: the compiler puts a wait before the first
+// use of each loaded value and the load counter drains in issue order, so
+// loading and using vector by vector cost two memory round trips per vector
+// (2^26 doubles, 10 exceptions per vector, staged: 155 us against 108 us
+// without exceptions).
+template <typename T, int LAYOUT, int PF>
+struct ListPrefetch {
+  using UT = typename uint_of<T>::type;
+  static constexpr int NL = lanes<UT>::N;
+  static constexpr int VPW = lanes<UT>::VPW;
+  uint64_t eo[VPW];
+  unsigned count[VPW];
+  unsigned pos[VPW][PF];  // raw positions (see entry_pos)
+  T val[VPW][PF];
+
+  __device__ __forceinline__ void load_headers(const AlpArgs<T> &a, uint64_t group) {
+    const uint64_t g = uniform_u64(group);
+    unsigned raw[VPW];
+#pragma unroll
+    for (int v2 = 0; v2 < VPW; ++v2) {
+      const uint64_t vec = g * VPW + v2;
+      const uint64_t vs = vec < a.ffor.n_vecs ? vec : 0;
+      eo[v2] = a.exceptions_offsets[vs];
+      if constexpr (LAYOUT == ALP_GALP) raw[v2] = a.offsets_counts[vs * NL + (NL - 1)];
+      else raw[v2] = a.counts[vs];
+    }
+#pragma unroll
+    for (int v2 = 0; v2 < VPW; ++v2) {
+      // G-ALP lanes are concatenated in order: the last lane's offset + count
+      const unsigned c = LAYOUT == ALP_GALP ? (raw[v2] & 0x3FFu) + (raw[v2] >> 10) : raw[v2];
+      count[v2] = __builtin_amdgcn_readfirstlane(g * VPW + v2 < a.ffor.n_vecs ? c : 0u);
+    }
+  }
+  // Entry t + 64 p of each list (see entry_pos), loaded with no branch (see
+  // list_entry).
+  __device__ __forceinline__ void load_entries(const AlpArgs<T> &a) {
+    const unsigned t = threadIdx.x % WAVE;
+#pragma unroll
+    for (int v2 = 0; v2 < VPW; ++v2) {
+#pragma unroll
+      for (int p = 0; p < PF; ++p) {
+        const unsigned k = t + unsigned(p * WAVE);
+        pos[v2][p] = *list_entry(a.positions, eo[v2], k, count[v2]);
+        val[v2][p] = *list_entry(a.exceptions, eo[v2], k, count[v2]);
+      }
+    }
+  }
+  // Raw position of entry t + 64 p of list v2, or 0xFFFF past its end.
+  __device__ __forceinline__ unsigned entry_pos(int v2, int p) const {
+    return (threadIdx.x % WAVE) + unsigned(p * WAVE) < count[v2] ? pos[v2][p] : 0xFFFFu;
+  }
+};
+
 // Per-wave LDS index of the standard-layout exception lists of the VPW
-// vectors of wave group `group`.
+// vectors of wave group `group`: a map of exception positions, a slot mask
+// per lane and the lists' first 64 values, so that an owner thread finds an
+// exception's value in LDS (its list index is its rank in the position map,
+// positions being sorted) and reads global memory only past entry 64.
 template <typename T>
 struct StdIndex {
   using UT = typename uint_of<T>::type;
@@ -181,21 +327,28 @@ struct StdIndex {
   static constexpr int VPW = lanes<UT>::VPW;
   uint64_t *bits;  // [VPW][16]: bit p set when position p holds an exception
   UT *lmask;       // [VPW][NL]: bit j set when slot j of the lane holds one
+  T *vals;         // [VPW][WAVE]: entries 0..63 of each list
 
   // All 64 threads of the wave must call this (no early exit before it).
   __device__ __forceinline__ void build(const AlpArgs<T> &a, uint64_t group) {
     const int t = threadIdx.x % WAVE;
+    ListPrefetch<T, ALP_STD, 1> lp;
+    lp.load_headers(a, group);
+    lp.load_entries(a);
     for (int i = t; i < VPW * 16; i += WAVE) bits[i] = 0;
     for (int i = t; i < VPW * NL; i += WAVE) lmask[i] = 0;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
+#pragma unroll
     for (int vv = 0; vv < VPW; ++vv) {
-      const uint64_t vec = group * VPW + vv;
-      if (vec >= a.ffor.n_vecs) break;
-      const unsigned count = a.counts[vec];
-      const uint64_t eo = a.exceptions_offsets[vec];
-      for (unsigned k = t; k < count; k += WAVE) {
-        const unsigned pos = a.positions[eo + k] & 1023u;
+      if (lp.entry_pos(vv, 0) != 0xFFFFu) {
+        const unsigned pos = lp.pos[vv][0] & 1023u;
+        lds_or<uint64_t>(&bits[vv * 16 + (pos >> 6)], uint64_t(1) << (pos & 63));
+        lds_or<UT>(&lmask[vv * NL + pos % NL], UT(UT(1) << (pos / NL)));
+        vals[vv * WAVE + t] = lp.val[vv][0];
+      }
+      for (unsigned k = unsigned(t + WAVE); k < lp.count[vv]; k += WAVE) {
+        const unsigned pos = a.positions[lp.eo[vv] + k] & 1023u;
         lds_or<uint64_t>(&bits[vv * 16 + (pos >> 6)], uint64_t(1) << (pos & 63));
         lds_or<UT>(&lmask[vv * NL + pos % NL], UT(UT(1) << (pos / NL)));
       }
@@ -218,6 +371,10 @@ struct StdIndex {
     for (unsigned w = 0; w < (p >> 6); ++w) r += unsigned(__builtin_popcountll(b[w]));
     return r;
   }
+  // Value of entry r of vector vv's list (vector `vec` of the column).
+  __device__ __forceinline__ T value(const AlpArgs<T> &a, uint64_t vec, int vv, unsigned r) const {
+    return r < unsigned(WAVE) ? vals[vv * WAVE + r] : a.exceptions[a.exceptions_offsets[vec] + r];
+  }
   // Exceptions of (vector vv of the group, lane) in slots [j0, j0 + nj).
   template <class F>
   __device__ __forceinline__ void own(const AlpArgs<T> &a, uint64_t vec, int vv, int lane, int j0,
@@ -228,32 +385,30 @@ struct StdIndex {
   template <class F>
   __device__ __forceinline__ void own_slots(const AlpArgs<T> &a, uint64_t vec, int vv, int lane,
                                             UT m, F &&f) const {
-    if (!m) return;
-    const uint64_t eo = a.exceptions_offsets[vec];
     while (m) {
-      unsigned pos[ALP_PF], rank[ALP_PF];
-      int n = 0;
-#pragma unroll
-      for (int i = 0; i < ALP_PF; ++i) {
-        if (m) {
-          const int j = int(__builtin_ctzll(uint64_t(m)));
-          m &= UT(m - 1);
-          const unsigned p = unsigned(lane + NL * j);
-          pos[i] = p;
-          rank[i] = this->rank(vv, p);
-          n = i + 1;
-        }
-      }
-      T val[ALP_PF];
-#pragma unroll
-      for (int i = 0; i < ALP_PF; ++i)
-        if (i < n) val[i] = a.exceptions[eo + rank[i]];
-#pragma unroll
-      for (int i = 0; i < ALP_PF; ++i)
-        if (i < n) f(pos[i], val[i]);
+      const int j = int(__builtin_ctzll(uint64_t(m)));
+      m &= UT(m - 1);
+      const unsigned p = unsigned(lane + NL * j);
+      f(p, value(a, vec, vv, rank(vv, p)));
     }
   }
 };
+
+// LDS for the standard-layout index of the calling wave (unused otherwise).
+template <typename T, int LAYOUT, bool PATCH>
+__device__ __forceinline__ StdIndex<T> wave_std_index() {
+  using UT = typename uint_of<T>::type;
+  using L = lanes<UT>;
+  if constexpr (PATCH && LAYOUT == ALP_STD) {
+    __shared__ uint64_t s_bits[BLOCK / WAVE][L::VPW * 16];
+    __shared__ UT s_lmask[BLOCK / WAVE][L::VPW * L::N];
+    __shared__ T s_vals[BLOCK / WAVE][L::VPW * WAVE];
+    const int w = threadIdx.x / WAVE;
+    return StdIndex<T>{s_bits[w], s_lmask[w], s_vals[w]};
+  } else {
+    return StdIndex<T>{nullptr, nullptr, nullptr};
+  }
+}
 
 // Per-thread scan of the exception list: the owner thread reads its lane's
 // exceptions one at a time (G-ALP) or scans the vector's list (standard
@@ -292,9 +447,7 @@ __device__ __forceinline__ void alp_decompress_unit_scan(const AlpArgs<T> &a, T 
   if (u.vec >= a.ffor.n_vecs) return;
   T *vout = out + u.vec * VALUES_PER_VECTOR;
   AlpDecompressPolicy<T, NT> pol{a.ffor.packed + a.ffor.vector_offsets[u.vec] + u.lane,
-                                 vout + u.lane, a.ffor.bases[u.vec],
-                                 alp_factor<T>(a.factor_indices[u.vec]),
-                                 alp_fraction<T>(a.fraction_indices[u.vec])};
+                                 vout + u.lane, alp_consts(a, u.vec)};
   if (!dispatch_bw<UT, 1>(pol, a.ffor.bit_widths[u.vec], 0)) {
     for (int j = 0; j < L::W; ++j) vout[u.lane + j * L::N] = T(0);
   }
@@ -311,23 +464,25 @@ __device__ __forceinline__ void alp_decompress_unit(const AlpArgs<T> &a, T *__re
   const int j0 = u.part * (L::W / S);
   constexpr int NJ = L::W / S;
   if constexpr (PATCH && LAYOUT == ALP_STD) {
-    __shared__ uint64_t s_bits[BLOCK / WAVE][L::VPW * 16];
-    __shared__ UT s_lmask[BLOCK / WAVE][L::VPW * L::N];
-    const int w = threadIdx.x / WAVE;
-    StdIndex<T> idx{s_bits[w], s_lmask[w]};
+    StdIndex<T> idx = wave_std_index<T, LAYOUT, PATCH>();
+    const bool live = u.vec < a.ffor.n_vecs;
+    const uint64_t vs = live ? u.vec : 0;
+    // the decode's metadata is loaded before the index's lists, so that one
+    // wait covers both
+    const uint64_t off = a.ffor.vector_offsets[vs];
+    const int bw = a.ffor.bit_widths[vs];
+    const UT base = a.ffor.bases[vs];
+    const unsigned fi = a.factor_indices[vs], ei = a.fraction_indices[vs];
     idx.build(a, wave / S);
-    if (u.vec >= a.ffor.n_vecs) return;
+    if (!live) return;
     T *vout = out + u.vec * VALUES_PER_VECTOR;
     const int vv = int((threadIdx.x % WAVE) / L::N);
-    AlpRegPatchPolicy<T, NT> pol{a.ffor.packed + a.ffor.vector_offsets[u.vec] + u.lane,
-                                 vout + u.lane, a.ffor.bases[u.vec],
-                                 alp_factor<T>(a.factor_indices[u.vec]),
-                                 alp_fraction<T>(a.fraction_indices[u.vec]), 0, {}};
+    AlpRegPatchPolicy<T, NT> pol{a.ffor.packed + off + u.lane, vout + u.lane,
+                                 AlpConsts<T>{base, fi, ei, 0, T(0)}, 0, {}};
     // the lane's first RK exception slots of the range go to the queue (list
     // index = rank in the position map), the rest are stored after the decode
     const UT lm = idx.slots(vv, u.lane, j0, NJ);
     UT rest = lm;
-    const uint64_t eo = lm ? a.exceptions_offsets[u.vec] : 0;
 #pragma unroll
     for (int i = 0; i < pol.RK; ++i) {
       pol.q[i] = T(0);
@@ -335,10 +490,10 @@ __device__ __forceinline__ void alp_decompress_unit(const AlpArgs<T> &a, T *__re
         const int j = int(__builtin_ctzll(uint64_t(rest)));
         rest &= UT(rest - 1);
         pol.emask |= uint64_t(1) << j;
-        pol.q[i] = a.exceptions[eo + idx.rank(vv, unsigned(u.lane + L::N * j))];
+        pol.q[i] = idx.value(a, u.vec, vv, idx.rank(vv, unsigned(u.lane + L::N * j)));
       }
     }
-    if (!dispatch_bw<UT, S>(pol, a.ffor.bit_widths[u.vec], u.part)) {
+    if (!dispatch_bw<UT, S>(pol, bw, u.part)) {
       for (int j = j0; j < j0 + NJ; ++j) vout[u.lane + j * L::N] = T(0);
       rest = lm;  // nothing was patched
     }
@@ -346,84 +501,66 @@ __device__ __forceinline__ void alp_decompress_unit(const AlpArgs<T> &a, T *__re
   } else if constexpr (PATCH) {
     if (u.vec >= a.ffor.n_vecs) return;
     T *vout = out + u.vec * VALUES_PER_VECTOR;
-    AlpRegPatchPolicy<T, NT> pol{a.ffor.packed + a.ffor.vector_offsets[u.vec] + u.lane,
-                                 vout + u.lane, a.ffor.bases[u.vec],
-                                 alp_factor<T>(a.factor_indices[u.vec]),
-                                 alp_fraction<T>(a.fraction_indices[u.vec]), 0, {}};
-    // the first RK entries of the lane's list: those in this thread's slot
-    // range go to the queue by slot rank (a repeated slot keeps the later
-    // entry); every other entry is stored after the decode
-    constexpr int RK = AlpRegPatchPolicy<T, NT>::RK;
+    const int bw = a.ffor.bit_widths[u.vec];
+    const uint64_t eo = a.exceptions_offsets[u.vec];
     const unsigned oc = a.offsets_counts[u.vec * L::N + u.lane];
-    const unsigned count = oc >> 10;
-    const uint64_t first = count ? a.exceptions_offsets[u.vec] + (oc & 0x3FFu) : 0;
-    unsigned slot[RK];
-    T val[RK];
-#pragma unroll
-    for (int k = 0; k < RK; ++k) {
-      slot[k] = 0xFFFFu;  // not queued
-      if (unsigned(k) < count) {
-        const unsigned pos = a.positions[first + k];
-        val[k] = a.exceptions[first + k];
-        const unsigned sl = pos / unsigned(L::N);
-        if (int(pos % unsigned(L::N)) == u.lane && sl >= unsigned(j0) && sl < unsigned(j0 + NJ))
-          slot[k] = sl;
-      }
-    }
+    AlpGalpRegPolicy<T, NT> pol;
+    pol.in = a.ffor.packed + a.ffor.vector_offsets[u.vec] + u.lane;
+    pol.out = vout + u.lane;
+    pol.c = alp_consts(a, u.vec);
+    pol.first = eo + (oc & 0x3FFu);
+    pol.count = oc >> 10;
+    pol.lane = u.lane;
+    pol.j0 = j0;
+    pol.nj = NJ;
+    pol.queued = 0;
+    pol.load_entries(a);
+    constexpr int RK = AlpGalpRegPolicy<T, NT>::RK;
+    const unsigned count = pol.count;
     uint64_t queued = 0;
-#pragma unroll
-    for (int k = 0; k < RK; ++k) {
-      pol.q[k] = T(0);
-      if (slot[k] != 0xFFFFu) {
-        pol.emask |= uint64_t(1) << slot[k];
-        queued |= uint64_t(1) << k;
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < RK; ++k) {
-      if (slot[k] != 0xFFFFu) {
-        const int r = __builtin_popcountll(pol.emask & ((uint64_t(1) << slot[k]) - 1));
-#pragma unroll
-        for (int i = 0; i < RK; ++i) pol.q[i] = r == i ? val[k] : pol.q[i];
-      }
-    }
-    if (!dispatch_bw<UT, S>(pol, a.ffor.bit_widths[u.vec], u.part)) {
+    const bool decoded = dispatch_bw<UT, S>(pol, bw, u.part);
+    if (decoded) {
+      queued = pol.queued;
+    } else {
       for (int j = j0; j < j0 + NJ; ++j) vout[u.lane + j * L::N] = T(0);
-      queued = 0;  // nothing was patched
     }
-    galp_own_exceptions<T>(a, u.vec, u.lane, j0, NJ,
-                           [&](unsigned pos, T e) { store<T, NT>(vout + pos, e); }, queued);
+    // every entry of a list of at most RK that lies in this thread's range
+    // is queued, so only longer lists (or an undecodable width) need the pass
+    if (count > unsigned(RK) || !decoded)
+      galp_own_exceptions<T>(a, u.vec, u.lane, j0, NJ,
+                             [&](unsigned pos, T e) { store<T, NT>(vout + pos, e); }, queued);
   } else {
     if (u.vec >= a.ffor.n_vecs) return;
     T *vout = out + u.vec * VALUES_PER_VECTOR;
     AlpDecompressPolicy<T, NT> pol{a.ffor.packed + a.ffor.vector_offsets[u.vec] + u.lane,
-                                   vout + u.lane, a.ffor.bases[u.vec],
-                                   alp_factor<T>(a.factor_indices[u.vec]),
-                                   alp_fraction<T>(a.fraction_indices[u.vec])};
+                                   vout + u.lane, alp_consts(a, u.vec)};
     if (!dispatch_bw<UT, S>(pol, a.ffor.bit_widths[u.vec], u.part)) {
       for (int j = j0; j < j0 + NJ; ++j) vout[u.lane + j * L::N] = T(0);
     }
   }
 }
 
-template <typename T>
+// `hook` runs once the packed words are in flight (see unpack_slots).
+template <typename T, class Hook = NoHook>
 struct AlpStagePolicy {
   using UT = typename uint_of<T>::type;
   const UT *in;
   T *lds;
-  UT base;
-  UT fact;
-  T frac;
+  AlpConsts<T> c;
+  Hook hook;
   template <int BW, int J0, int NJ>
   __device__ __forceinline__ void run() {
-    unpack_slots<UT, BW, J0, NJ>(in, [&](int j, UT v) {
-      lds[(j - J0) * lanes<UT>::N] = alp_value<T>(v, base, fact, frac);
-    });
+    unpack_slots<UT, BW, J0, NJ>(
+        in, [&](int j, UT v) { lds[(j - J0) * lanes<UT>::N] = c(v); },
+        [&] {
+          c.resolve();
+          hook();
+        });
   }
   __device__ __forceinline__ void run_generic(int bw, int j0, int nj) {
-    unpack_runtime<UT>(in, bw, j0, nj, [&](int j, UT v) {
-      lds[(j - j0) * lanes<UT>::N] = alp_value<T>(v, base, fact, frac);
-    });
+    c.resolve();
+    hook();
+    unpack_runtime<UT>(in, bw, j0, nj, [&](int j, UT v) { lds[(j - j0) * lanes<UT>::N] = c(v); });
   }
 };
 
@@ -443,49 +580,29 @@ __global__ __launch_bounds__(BLOCK) void k_alp_decompress_staged(AlpArgs<T> a, T
   const int t = int(threadIdx.x % WAVE), vv = t / L::N;
   const int j0 = u.part * Sh::NJ;
   const uint64_t group = wave / S;
-  // The first PF * 64 exceptions of each vector are loaded before the decode
-  // so that their latency overlaps it; longer lists finish in a loop after.
+  // The first PF * 64 exceptions of each vector are loaded while the packed
+  // words are in flight (the decode's hook), so that one memory round trip
+  // covers both; longer lists finish in a loop after the decode. Thread t
+  // loads entry t of every vector of the group, so a thread whose own vector
+  // does not exist, or does not decode, still loads its entries.
   constexpr int PF = PATCH ? (sizeof(T) == 4 ? 3 : 1) : 1;
-  uint64_t eo[L::VPW];
-  unsigned count[L::VPW];
-  unsigned pf_pos[L::VPW][PF];
-  T pf_val[L::VPW][PF];
-  if constexpr (PATCH) {
-#pragma unroll
-    for (int v2 = 0; v2 < L::VPW; ++v2) {
-      const uint64_t vec = group * L::VPW + v2;
-      count[v2] = 0;
-      eo[v2] = 0;
-      if (vec < a.ffor.n_vecs) {
-        eo[v2] = a.exceptions_offsets[vec];
-        if constexpr (LAYOUT == ALP_GALP) {
-          // lanes are concatenated in order: the last lane's offset + count
-          const unsigned oc = a.offsets_counts[vec * L::N + (L::N - 1)];
-          count[v2] = (oc & 0x3FFu) + (oc >> 10);
-        } else {
-          count[v2] = a.counts[vec];
-        }
-      }
-#pragma unroll
-      for (int p = 0; p < PF; ++p) {
-        const unsigned k = unsigned(t + p * WAVE);
-        pf_pos[v2][p] = 0xFFFFu;  // never inside a slot range
-        if (k < count[v2]) {
-          pf_pos[v2][p] = a.positions[eo[v2] + k] & 1023u;
-          pf_val[v2][p] = a.exceptions[eo[v2] + k];
-        }
-      }
-    }
+  ListPrefetch<T, LAYOUT, PF> lp;
+  if constexpr (PATCH) lp.load_headers(a, group);
+  bool entries = !PATCH;
+  auto hook = [&] {
+    if constexpr (PATCH) lp.load_entries(a);
+    entries = true;
+  };
+  // A lane past the column's last vector decodes vector 0 into its unused
+  // LDS row, so the code has no branch between the loads (store_staged skips
+  // that row).
+  const uint64_t vs = u.vec < a.ffor.n_vecs ? u.vec : 0;
+  AlpStagePolicy<T, decltype(hook) &> pol{a.ffor.packed + a.ffor.vector_offsets[vs] + u.lane,
+                                          lds + vv * Sh::CHUNK + u.lane, alp_consts(a, vs), hook};
+  if (!dispatch_bw<UT, S>(pol, a.ffor.bit_widths[vs], u.part)) {
+    for (int jj = 0; jj < Sh::NJ; ++jj) lds[vv * Sh::CHUNK + u.lane + jj * L::N] = T(0);
   }
-  if (u.vec < a.ffor.n_vecs) {
-    AlpStagePolicy<T> pol{a.ffor.packed + a.ffor.vector_offsets[u.vec] + u.lane,
-                          lds + vv * Sh::CHUNK + u.lane, a.ffor.bases[u.vec],
-                          alp_factor<T>(a.factor_indices[u.vec]),
-                          alp_fraction<T>(a.fraction_indices[u.vec])};
-    if (!dispatch_bw<UT, S>(pol, a.ffor.bit_widths[u.vec], u.part)) {
-      for (int jj = 0; jj < Sh::NJ; ++jj) lds[vv * Sh::CHUNK + u.lane + jj * L::N] = T(0);
-    }
-  }
+  if (!entries) hook();
   if constexpr (PATCH) {
     wave_lds_sync();
     const unsigned lo = unsigned(L::N * j0), hi = unsigned(L::N * (j0 + Sh::NJ));
@@ -493,12 +610,13 @@ __global__ __launch_bounds__(BLOCK) void k_alp_decompress_staged(AlpArgs<T> a, T
     for (int v2 = 0; v2 < L::VPW; ++v2) {
 #pragma unroll
       for (int p = 0; p < PF; ++p) {
-        const unsigned pos = pf_pos[v2][p];
-        if (pos >= lo && pos < hi) lds[v2 * Sh::CHUNK + (pos - lo)] = pf_val[v2][p];
+        const unsigned pos = lp.pos[v2][p] & 1023u;
+        if (lp.entry_pos(v2, p) != 0xFFFFu && pos >= lo && pos < hi)
+          lds[v2 * Sh::CHUNK + (pos - lo)] = lp.val[v2][p];
       }
-      for (unsigned k = unsigned(t + PF * WAVE); k < count[v2]; k += WAVE) {
-        const unsigned pos = a.positions[eo[v2] + k] & 1023u;
-        if (pos >= lo && pos < hi) lds[v2 * Sh::CHUNK + (pos - lo)] = a.exceptions[eo[v2] + k];
+      for (unsigned k = unsigned(t + PF * WAVE); k < lp.count[v2]; k += WAVE) {
+        const unsigned pos = a.positions[lp.eo[v2] + k] & 1023u;
+        if (pos >= lo && pos < hi) lds[v2 * Sh::CHUNK + (pos - lo)] = a.exceptions[lp.eo[v2] + k];
       }
     }
   }
@@ -520,9 +638,8 @@ __device__ __forceinline__ uint64_t alp_lane_match(const AlpArgs<T> &a, uint64_t
                                                    T target, const StdIndex<T> *idx) {
   using UT = typename uint_of<T>::type;
   using L = lanes<UT>;
-  AlpMatchPolicy<T> pol{a.ffor.packed + a.ffor.vector_offsets[vec] + lane, a.ffor.bases[vec],
-                        alp_factor<T>(a.factor_indices[vec]),
-                        alp_fraction<T>(a.fraction_indices[vec]), target, 0};
+  AlpMatchPolicy<T> pol{a.ffor.packed + a.ffor.vector_offsets[vec] + lane, alp_consts(a, vec),
+                        target, 0};
   if (!dispatch_bw<UT, 1>(pol, a.ffor.bit_widths[vec], 0)) {
     constexpr uint64_t FULL = L::W == 64 ? ~uint64_t(0) : ((uint64_t(1) << L::W) - 1);
     pol.mask = T(0) == target ? FULL : 0;
@@ -537,21 +654,6 @@ __device__ __forceinline__ uint64_t alp_lane_match(const AlpArgs<T> &a, uint64_t
     else idx->own(a, vec, int((threadIdx.x % WAVE) / L::N), lane, 0, L::W, patch);
   }
   return m;
-}
-
-// LDS for the standard-layout index of the calling wave (unused otherwise).
-template <typename T, int LAYOUT, bool PATCH>
-__device__ __forceinline__ StdIndex<T> wave_std_index() {
-  using UT = typename uint_of<T>::type;
-  using L = lanes<UT>;
-  if constexpr (PATCH && LAYOUT == ALP_STD) {
-    __shared__ uint64_t s_bits[BLOCK / WAVE][L::VPW * 16];
-    __shared__ UT s_lmask[BLOCK / WAVE][L::VPW * L::N];
-    const int w = threadIdx.x / WAVE;
-    return StdIndex<T>{s_bits[w], s_lmask[w]};
-  } else {
-    return StdIndex<T>{nullptr, nullptr};
-  }
 }
 
 // "Any value == magic" over a patched column. The patched column holds the
@@ -585,8 +687,7 @@ __global__ __launch_bounds__(BLOCK) void k_alp_any(AlpArgs<T> a, T magic, uint64
     if (u.vec >= a.ffor.n_vecs) continue;
     const int j0 = u.part * NJ;
     AlpMatchPolicy<T> pol{a.ffor.packed + a.ffor.vector_offsets[u.vec] + u.lane,
-                          a.ffor.bases[u.vec], alp_factor<T>(a.factor_indices[u.vec]),
-                          alp_fraction<T>(a.fraction_indices[u.vec]), magic, 0};
+                          alp_consts(a, u.vec), magic, 0};
     if (!dispatch_bw<UT, S>(pol, a.ffor.bit_widths[u.vec], u.part))
       pol.mask = T(0) == magic ? FULL : 0;
     uint64_t m = pol.mask;  // bit (j - j0): slot j decodes to magic

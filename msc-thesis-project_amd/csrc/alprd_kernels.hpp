@@ -25,6 +25,35 @@ namespace flsgpu_hip {
 // 16 bits of `lq` (the lane's first RD_REG exceptions in slot order), so
 // the value is stored once, already patched.
 constexpr int RD_REG = 4;
+// Exception entries per vector staged in LDS by the one-thread-per-lane
+// decoder (RdListPolicy).
+constexpr unsigned RD_LIST = 64;
+
+// The Q uint16 left-index lanes a thread of the W-bit layout needs, each
+// later held in one 64-bit register (left_bw <= 3 words of 16 bits). The
+// loads are issued with no branch: words past left_bw re-read word 0, and
+// left_compose masks them off.
+template <typename UT>
+struct LeftWords {
+  static constexpr int Q = 64 / lanes<UT>::N;
+  uint16_t w[Q][3];
+  __device__ __forceinline__ void load(const uint16_t *lin, int lane, unsigned lbw) {
+    constexpr int NL = lanes<UT>::N;
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+      for (unsigned r = 0; r < 3; ++r) w[q][r] = lin[lane + NL * q + 64 * (r < lbw ? r : 0)];
+  }
+  __device__ __forceinline__ void compose(unsigned lbw, uint64_t *lstream) const {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      uint64_t x = 0;
+#pragma unroll
+      for (unsigned r = 0; r < 3; ++r) x |= r < lbw ? uint64_t(w[q][r]) << (16 * r) : 0;
+      lstream[q] = x;
+    }
+  }
+};
 
 template <typename T, bool NT>
 struct RdPolicy {
@@ -38,18 +67,95 @@ struct RdPolicy {
   unsigned lbw;
   uint64_t emask;
   uint64_t lq;
+  template <int BW, class Hook = NoHook>
+  __device__ __forceinline__ void decode(Hook &&after_loads = Hook{}) {
+    const uint64_t lmask = (uint64_t(1) << lbw) - 1;
+    unpack_slots<UT, BW, 0, lanes<UT>::W>(
+        rin,
+        [&](int j, UT r) {
+          const unsigned idx = unsigned(lstream[j % Q] >> ((j / Q) * lbw)) & unsigned(lmask);
+          const bool ex = (emask >> j) & 1u;
+          const UT v = ex ? UT(UT(UT(lq & 0xFFFFu) << BW) | r) : UT(dshift[idx] | r);
+          lq = ex ? (lq >> 16) : lq;
+          store<T, NT>(out + j * NL, __builtin_bit_cast(T, v));
+        },
+        after_loads);
+  }
   template <int BW, int J0, int NJ>
   __device__ __forceinline__ void run() {
-    const uint64_t lmask = (uint64_t(1) << lbw) - 1;
-    unpack_slots<UT, BW, J0, NJ>(rin, [&](int j, UT r) {
-      const unsigned idx = unsigned(lstream[j % Q] >> ((j / Q) * lbw)) & unsigned(lmask);
-      const bool ex = (emask >> j) & 1u;
-      const UT v = ex ? UT(UT(UT(lq & 0xFFFFu) << BW) | r) : UT(dshift[idx] | r);
-      lq = ex ? (lq >> 16) : lq;
-      store<T, NT>(out + j * NL, __builtin_bit_cast(T, v));
-    });
+    decode<BW>();
   }
 };
+
+// One thread per lane with the vector's exception list handed over in LDS:
+// the vector's N_LANES threads load its first RD_LIST entries (64 / N_LANES
+// each) once the right-part words are in flight (prepare, the after-loads
+// hook), write them to the wave's LDS list as (left << 16) | position, and
+// each thread collects its lane's entries: the first RD_REG (in slot order)
+// into the register queue, the rest are stored after the decode.
+template <typename T, bool NT>
+struct RdListPolicy : RdPolicy<T, NT> {
+  using UT = typename uint_of<T>::type;
+  static constexpr int NL = lanes<UT>::N;
+  static constexpr int PFN = int(RD_LIST) / NL;
+  const uint16_t *positions;
+  const uint16_t *exceptions;
+  uint64_t eo;
+  unsigned count, n_list;
+  int lane;
+  uint32_t *list;         // this wave's LDS list
+  const uint32_t *vlist;  // this vector's part of it
+  uint64_t own;
+  LeftWords<UT> left;
+  __device__ __forceinline__ void prepare() {
+    left.compose(this->lbw, this->lstream);
+    uint16_t p[PFN], l[PFN];
+#pragma unroll
+    for (int i = 0; i < PFN; ++i) {
+      const unsigned k = unsigned(lane + NL * i);
+      p[i] = *list_entry(positions, eo, k, count);
+      l[i] = *list_entry(exceptions, eo, k, count);
+    }
+#pragma unroll
+    for (int i = 0; i < PFN; ++i)
+      list[(vlist - list) + lane + NL * i] = (p[i] & 1023u) | (uint32_t(l[i]) << 16);
+    wave_lds_sync();
+    own = 0;
+    for (unsigned k = 0; k < n_list; ++k) {
+      const unsigned pos = vlist[k] & 1023u;
+      if (int(pos % NL) == lane) own |= uint64_t(1) << (pos / NL);
+    }
+    uint64_t em = own;
+    for (int extra = __builtin_popcountll(own) - RD_REG; extra > 0; --extra)
+      em &= ~(uint64_t(1) << (63 - __builtin_clzll(em)));
+    uint64_t lq = 0;
+    if (em) {
+      for (unsigned k = 0; k < n_list; ++k) {
+        const uint32_t e = vlist[k];
+        const unsigned pos = e & 1023u;
+        const unsigned slot = pos / NL;
+        if (int(pos % NL) != lane || !((em >> slot) & 1u)) continue;
+        const int rank = __builtin_popcountll(em & ((uint64_t(1) << slot) - 1));
+        lq = (lq & ~(uint64_t(0xFFFFu) << (16 * rank))) | (uint64_t(e >> 16) << (16 * rank));
+      }
+    }
+    this->emask = em;
+    this->lq = lq;
+  }
+  template <int BW, int J0, int NJ>
+  __device__ __forceinline__ void run() {
+    this->template decode<BW>([&] { prepare(); });
+  }
+};
+
+// The block fills the pre-shifted dictionary in LDS before any wave decodes.
+template <typename T>
+__device__ __forceinline__ void alprd_fill_dictionary(const AlprdArgs<T> &a,
+                                                      typename uint_of<T>::type *dshift) {
+  using UT = typename uint_of<T>::type;
+  if (threadIdx.x < 8)
+    dshift[threadIdx.x] = threadIdx.x < a.dict_size ? UT(UT(a.dict[threadIdx.x]) << a.right_bw) : UT(0);
+}
 
 // right_bw is one value per column, so the dispatch is a scalar switch.
 template <typename UT, class Policy>
@@ -68,30 +174,20 @@ __device__ __forceinline__ bool dispatch_right_bw(Policy &pol, int bw) {
   }
 }
 
-// The block fills the pre-shifted dictionary in LDS before any wave decodes.
-template <typename T>
-__device__ __forceinline__ void alprd_fill_dictionary(const AlprdArgs<T> &a,
-                                                      typename uint_of<T>::type *dshift) {
-  using UT = typename uint_of<T>::type;
-  if (threadIdx.x < 8)
-    dshift[threadIdx.x] = threadIdx.x < a.dict_size ? UT(UT(a.dict[threadIdx.x]) << a.right_bw) : UT(0);
-}
+// Reading each thread's whole list from global memory and patching after
+// the decode made the time grow with the exception count (2^26 doubles: 201
+// us at 4 exceptions per vector, 289 us at 18); the LDS hand-over and the
+// register queue keep it flat.
+// Where lanes past a column's last vector write (see alprd_decompress_unit).
+// Never read.
+static __device__ uint64_t k_rd_sink[1024];
 
-// Exception entries per vector that the one-thread-per-lane decoder stages in
-// LDS: the vector's N_LANES threads load the first RD_LIST entries of its
-// list together (64 / N_LANES each, coalesced) as (left << 16) | position.
-// Every thread then reads the staged list for its lane's entries: the first
-// RD_REG (in slot order) are patched in registers during the decode; any
-// further ones, and entries past RD_LIST, are stored after the decode (the
-// same thread's later store to the same address wins). Reading the whole
-// list from global memory per thread and patching after the decode made the
-// time grow with the exception count (2^26 doubles: 201 us at 4 exceptions
-// per vector, 289 us at 18).
-constexpr unsigned RD_LIST = 64;
-
-// `list`: this wave's LDS, VPW * RD_LIST entries. Every lane of the wave
-// calls this (the LDS hand-over is wave-wide); lanes past the column end
-// decode nothing.
+// `list`: this wave's LDS, VPW * RD_LIST entries. The right-part words'
+// addresses need no metadata, so they are the first loads the decode issues;
+// the list is set up in its after-loads hook, whose LDS hand-over is
+// wave-wide. Every lane of the wave takes the same path through it: a lane
+// past the column's last vector decodes vector 0 with no exceptions into a
+// sink that nothing reads.
 template <typename T, bool NT>
 __device__ __forceinline__ void alprd_decompress_unit(const AlprdArgs<T> &a, T *__restrict out,
                                                       uint64_t wave,
@@ -99,73 +195,36 @@ __device__ __forceinline__ void alprd_decompress_unit(const AlprdArgs<T> &a, T *
                                                       uint32_t *list) {
   using UT = typename uint_of<T>::type;
   using L = lanes<UT>;
-  constexpr int Q = 64 / L::N;
-  constexpr int PFN = int(RD_LIST) / L::N;
   const UnitMap<UT, 1> u = map_unit<UT, 1>(wave);
   const int vv = int(threadIdx.x % WAVE) / L::N;
-  bool live = u.vec < a.n_vecs;
-  uint64_t eo = 0;
-  unsigned count = 0;
-  uint32_t ent[PFN];
-#pragma unroll
-  for (int i = 0; i < PFN; ++i) ent[i] = 0;
-  T *vout = out + u.vec * VALUES_PER_VECTOR;
-  const UT *rin = a.right + u.vec * uint64_t(a.right_bw) * L::N + u.lane;
-  RdPolicy<T, NT> pol;
-  if (live) {
-    eo = a.exceptions_offsets[u.vec];
-    count = a.counts[u.vec];
-#pragma unroll
-    for (int i = 0; i < PFN; ++i) {
-      const unsigned k = unsigned(u.lane + L::N * i);
-      if (k < count)
-        ent[i] = (a.positions[eo + k] & 1023u) | (uint32_t(a.exceptions[eo + k]) << 16);
-    }
-    const uint16_t *lin = a.left + u.vec * uint64_t(a.left_bw) * 64;
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      uint64_t s = 0;
-      for (unsigned r = 0; r < a.left_bw; ++r)
-        s |= uint64_t(lin[u.lane + L::N * q + 64 * r]) << (16 * r);
-      pol.lstream[q] = s;
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < PFN; ++i) list[vv * RD_LIST + u.lane + L::N * i] = ent[i];
-  wave_lds_sync();
-  const uint32_t *vlist = list + vv * RD_LIST;
-  const unsigned n_list = count < RD_LIST ? count : RD_LIST;
-  // this lane's exception slots among the staged entries
-  uint64_t own = 0;
-  for (unsigned k = 0; k < n_list; ++k) {
-    const unsigned pos = vlist[k] & 1023u;
-    if (int(pos % L::N) == u.lane) own |= uint64_t(1) << (pos / L::N);
-  }
-  // the first RD_REG of them go to registers, in slot order
-  uint64_t emask = own;
-  for (int extra = __builtin_popcountll(own) - RD_REG; extra > 0; --extra)
-    emask &= ~(uint64_t(1) << (63 - __builtin_clzll(emask)));
-  uint64_t lq = 0;
-  if (emask) {
-    for (unsigned k = 0; k < n_list; ++k) {
-      const uint32_t e = vlist[k];
-      const unsigned pos = e & 1023u;
-      const unsigned slot = pos / L::N;
-      if (int(pos % L::N) != u.lane || !((emask >> slot) & 1u)) continue;
-      const int rank = __builtin_popcountll(emask & ((uint64_t(1) << slot) - 1));
-      lq = (lq & ~(uint64_t(0xFFFFu) << (16 * rank))) | (uint64_t(e >> 16) << (16 * rank));
-    }
-  }
-  if (live) {
-    pol.rin = rin;
-    pol.out = vout + u.lane;
-    pol.dshift = dshift;
-    pol.lbw = a.left_bw;
-    pol.emask = emask;
-    pol.lq = lq;
-    live = dispatch_right_bw<UT>(pol, int(a.right_bw));  // false only for widths rejected on the host
-  }
-  if (!live) return;
+  const bool live = u.vec < a.n_vecs;
+  const uint64_t vs = live ? u.vec : 0;
+  const uint64_t eo = a.exceptions_offsets[vs];
+  const unsigned count = live ? unsigned(a.counts[vs]) : 0u;
+  T *vout = live ? out + u.vec * VALUES_PER_VECTOR : reinterpret_cast<T *>(k_rd_sink);
+  const UT *rin = a.right + vs * uint64_t(a.right_bw) * L::N + u.lane;
+  RdListPolicy<T, NT> pol;
+  pol.rin = rin;
+  pol.out = vout + u.lane;
+  pol.dshift = dshift;
+  pol.lbw = a.left_bw;
+  pol.emask = 0;
+  pol.lq = 0;
+  pol.positions = a.positions;
+  pol.exceptions = a.exceptions;
+  pol.eo = eo;
+  pol.count = count;
+  pol.n_list = count < RD_LIST ? count : RD_LIST;
+  pol.lane = u.lane;
+  pol.list = list;
+  pol.vlist = list + vv * RD_LIST;
+  pol.own = 0;
+  pol.left.load(a.left + vs * uint64_t(a.left_bw) * 64, u.lane, a.left_bw);
+  if (!dispatch_right_bw<UT>(pol, int(a.right_bw))) return;  // widths are checked on the host
+  const uint64_t own = pol.own;
+  const unsigned n_list = pol.n_list;
+  const uint32_t *vlist = pol.vlist;
+  uint64_t emask = pol.emask;
   // entries the registers did not take: stored over the decoded values
   if (own != emask) {
     for (unsigned k = 0; k < n_list; ++k) {
@@ -209,7 +268,8 @@ __global__ __launch_bounds__(BLOCK) void k_alprd_decompress(AlprdArgs<T> a, T *_
                                s_list[threadIdx.x / WAVE]);
 }
 
-template <typename T>
+// `hook` runs once the right-part words are in flight (see unpack_slots).
+template <typename T, class Hook>
 struct RdStagePolicy {
   using UT = typename uint_of<T>::type;
   static constexpr int NL = lanes<UT>::N;
@@ -219,13 +279,21 @@ struct RdStagePolicy {
   const UT *dshift;
   uint64_t lstream[Q];
   unsigned lbw;
+  LeftWords<UT> left;
+  Hook hook;
   template <int BW, int J0, int NJ>
   __device__ __forceinline__ void run() {
     const uint64_t lmask = (uint64_t(1) << lbw) - 1;
-    unpack_slots<UT, BW, J0, NJ>(rin, [&](int j, UT r) {
-      const unsigned idx = unsigned(lstream[j % Q] >> ((j / Q) * lbw)) & unsigned(lmask);
-      lds[(j - J0) * NL] = __builtin_bit_cast(T, UT(dshift[idx] | r));
-    });
+    unpack_slots<UT, BW, J0, NJ>(
+        rin,
+        [&](int j, UT r) {
+          const unsigned idx = unsigned(lstream[j % Q] >> ((j / Q) * lbw)) & unsigned(lmask);
+          lds[(j - J0) * NL] = __builtin_bit_cast(T, UT(dshift[idx] | r));
+        },
+        [&] {
+          left.compose(lbw, lstream);
+          hook();
+        });
   }
 };
 
@@ -257,7 +325,6 @@ __global__ __launch_bounds__(BLOCK) void k_alprd_decompress_staged(AlprdArgs<T> 
   using UT = typename uint_of<T>::type;
   using L = lanes<UT>;
   using Sh = StageShape<UT, S>;
-  constexpr int Q = 64 / L::N;
   __shared__ UT dshift[8];
   __shared__ __align__(16) T s_stage[BLOCK / WAVE][L::VPW * Sh::CHUNK];
   alprd_fill_dictionary<T>(a, dshift);
@@ -268,41 +335,45 @@ __global__ __launch_bounds__(BLOCK) void k_alprd_decompress_staged(AlprdArgs<T> 
   const int t = int(threadIdx.x % WAVE), vv = t / L::N;
   const int j0 = u.part * Sh::NJ;
   const uint64_t group = wave / S;
-  // the first 64 exceptions of each vector, loaded before the decode
+  // The headers of the group's vectors first (wave-uniform), then each
+  // vector's first 64 exceptions (entry t to thread t) once the right-part
+  // words are in flight, so one round trip covers both.
+  const uint64_t g = uniform_u64(group);
   uint64_t eo[L::VPW];
   unsigned count[L::VPW], pf_pos[L::VPW];
   uint16_t pf_left[L::VPW];
 #pragma unroll
   for (int v2 = 0; v2 < L::VPW; ++v2) {
-    const uint64_t vec = group * L::VPW + v2;
-    count[v2] = 0;
-    eo[v2] = 0;
-    pf_pos[v2] = 0xFFFFu;  // never inside a slot range
-    if (vec < a.n_vecs) {
-      eo[v2] = a.exceptions_offsets[vec];
-      count[v2] = a.counts[vec];
-      if (unsigned(t) < count[v2]) {
-        pf_pos[v2] = a.positions[eo[v2] + t] & 1023u;
-        pf_left[v2] = a.exceptions[eo[v2] + t];
-      }
-    }
+    const uint64_t vec = g * L::VPW + v2;
+    const uint64_t vs = vec < a.n_vecs ? vec : 0;
+    eo[v2] = a.exceptions_offsets[vs];
+    count[v2] = a.counts[vs];
   }
-  if (u.vec < a.n_vecs) {
-    RdStagePolicy<T> pol;
-    pol.rin = a.right + u.vec * uint64_t(a.right_bw) * L::N + u.lane;
-    pol.lds = lds + vv * Sh::CHUNK + u.lane;
-    pol.dshift = dshift;
-    pol.lbw = a.left_bw;
-    const uint16_t *lin = a.left + u.vec * uint64_t(a.left_bw) * 64;
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      uint64_t s = 0;
-      for (unsigned r = 0; r < a.left_bw; ++r)
-        s |= uint64_t(lin[u.lane + L::N * q + 64 * r]) << (16 * r);
-      pol.lstream[q] = s;
+  for (int v2 = 0; v2 < L::VPW; ++v2)
+    count[v2] = __builtin_amdgcn_readfirstlane(g * L::VPW + v2 < a.n_vecs ? count[v2] : 0u);
+  bool entries = false;
+  auto hook = [&] {
+#pragma unroll
+    for (int v2 = 0; v2 < L::VPW; ++v2) {
+      pf_pos[v2] = *list_entry(a.positions, eo[v2], unsigned(t), count[v2]);
+      pf_left[v2] = *list_entry(a.exceptions, eo[v2], unsigned(t), count[v2]);
     }
-    dispatch_right_bw_part<UT, S>(pol, int(a.right_bw), u.part);  // widths checked on the host
-  }
+    entries = true;
+  };
+  // a lane past the column's last vector decodes vector 0 into its unused
+  // LDS row (store_staged skips it), so there is no branch between the loads
+  const uint64_t vs = u.vec < a.n_vecs ? u.vec : 0;
+  RdStagePolicy<T, decltype(hook) &> pol{a.right + vs * uint64_t(a.right_bw) * L::N + u.lane,
+                                         lds + vv * Sh::CHUNK + u.lane, dshift, {}, a.left_bw,
+                                         {}, hook};
+  pol.left.load(a.left + vs * uint64_t(a.left_bw) * 64, u.lane, a.left_bw);
+  dispatch_right_bw_part<UT, S>(pol, int(a.right_bw), u.part);  // widths checked on the host
+  if (!entries) hook();
+#pragma unroll
+  for (int v2 = 0; v2 < L::VPW; ++v2)
+    if (unsigned(t) >= count[v2]) pf_pos[v2] = 0xFFFFu;  // never inside a slot range
+    else pf_pos[v2] &= 1023u;
   wave_lds_sync();
   const unsigned lo = unsigned(L::N * j0), hi = unsigned(L::N * (j0 + Sh::NJ));
   const UT rmask = UT((UT(1) << a.right_bw) - 1);

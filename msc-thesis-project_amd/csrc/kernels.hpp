@@ -37,6 +37,29 @@ __host__ __device__ constexpr uint64_t n_waves_for(uint64_t n_vecs) {
   return ((n_vecs + lanes<UT>::VPW - 1) / lanes<UT>::VPW) * S;
 }
 
+// A value the caller knows to be the same in every lane of the wave, moved
+// to scalar registers so that loads indexed by it are scalar loads.
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(x));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(x >> 32));
+  return (uint64_t(hi) << 32) | lo;
+}
+
+// Stands in for the exception arrays of an empty list (see list_entry).
+static __device__ uint64_t k_no_exception[1];  // zero; never written
+
+// Address of entry k of an exception list of `count` entries starting at
+// `first`: past the list's end its first entry, and for an empty list a zero
+// stub, so a thread can load an entry with no branch and never reads outside
+// the arrays. (A branch between a load and its use makes the compiler's wait
+// before the use also wait for every load issued since, and the load
+// counter drains in issue order.)
+template <typename E>
+__device__ __forceinline__ const E *list_entry(const E *arr, uint64_t first, unsigned k,
+                                               unsigned count) {
+  return count == 0 ? reinterpret_cast<const E *>(k_no_exception) : arr + first + (k < count ? k : 0);
+}
+
 enum class RedOp { Add, Or, And };
 
 template <RedOp OP>

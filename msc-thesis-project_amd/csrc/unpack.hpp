@@ -32,14 +32,24 @@ __device__ __forceinline__ void static_for(F &&f) {
   static_for_impl(std::make_integer_sequence<int, N>{}, f);
 }
 
+struct NoHook {
+  __device__ __forceinline__ void operator()() const {}
+};
+
 // Calls emit(j, value) for slots J0..J0+NJ-1 of one lane. `in` points at the
-// lane's word 0 of the vector (packed + vector_offset + lane).
-template <typename UT, int BW, int J0, int NJ, class Emit>
-__device__ __forceinline__ void unpack_slots(const UT *__restrict in, Emit &&emit) {
+// lane's word 0 of the vector (packed + vector_offset + lane). `after_loads`
+// runs once every packed word load has been issued and before any is used:
+// loads it issues are in flight together with the packed words, and waiting
+// for the packed words does not wait for them first (the load counter drains
+// in issue order).
+template <typename UT, int BW, int J0, int NJ, class Emit, class Hook = NoHook>
+__device__ __forceinline__ void unpack_slots(const UT *__restrict in, Emit &&emit,
+                                             Hook &&after_loads = Hook{}) {
   constexpr int W = lanes<UT>::W;
   constexpr int NL = lanes<UT>::N;
   static_assert(BW >= 0 && BW <= W, "bit width out of range");
   if constexpr (BW == 0) {
+    after_loads();
     static_for<NJ>([&](auto jc) { emit(J0 + int(decltype(jc)::value), UT(0)); });
   } else {
     constexpr int R0 = (J0 * BW) / W;
@@ -51,6 +61,7 @@ __device__ __forceinline__ void unpack_slots(const UT *__restrict in, Emit &&emi
       constexpr int r = decltype(rc)::value;
       w[r] = in[(R0 + r) * NL];
     });
+    after_loads();
     static_for<NJ>([&](auto jc) {
       constexpr int j = J0 + int(decltype(jc)::value);
       constexpr int p = j * BW;

@@ -151,6 +151,9 @@ def ref():
             f = getattr(_ref, f"ref_alprd_find_dictionary_{sfx}")
             f.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p]
             f.restype = C.c_uint
+            f = getattr(_ref, f"ref_alp_column_decode_{sfx}")
+            f.argtypes = [C.c_size_t, C.c_size_t] + [C.c_void_p] * 11
+            f.restype = None
         _ref.ref_alprd_decode_vec_f64.argtypes = [C.c_void_p] * 5 + [C.c_uint16, C.c_void_p, C.c_uint, C.c_uint]
         _ref.ref_alprd_decode_vec_f64.restype = None
     return _ref
@@ -270,6 +273,20 @@ def alp_decode_vec(packed, bw, base, fac, exp, dtype, use_ref=False):
     else:
         getattr(lib(), f"orc_alp_decode_vec_{sfx}")(_p(src), _p(out), bw, int(base), fac, exp)
     return out
+
+
+def ref_alp_column_decode_range(col, v0: int, v1: int, out: np.ndarray) -> None:
+    """The compiled reference's CPU decode (falp + patch_exceptions per
+    vector, src/alp/alp-bindings.cu:208-228) of vectors [v0, v1) of a
+    standard-layout ALP column into out[v0 * 1024 : v1 * 1024]. `col` is any
+    object with the ALPColumn array fields at top level (packed, bit_widths,
+    vector_offsets, bases, factor_indices, fraction_indices, exceptions,
+    positions, exceptions_offsets, counts)."""
+    sfx = "f32" if out.dtype == np.float32 else "f64"
+    getattr(ref(), f"ref_alp_column_decode_{sfx}")(
+        v0, v1, _p(col.packed), _p(col.bit_widths), _p(col.vector_offsets), _p(col.bases),
+        _p(col.factor_indices), _p(col.fraction_indices), _p(col.exceptions), _p(col.positions),
+        _p(col.exceptions_offsets), _p(col.counts), _p(out))
 
 
 def alp_column_decode(col: ALPColumn) -> np.ndarray:

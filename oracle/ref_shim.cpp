@@ -70,6 +70,27 @@ void ref_patch_f64(double *out, const double *exc, const uint16_t *pos, uint16_t
   alp::decoder<double>::patch_exceptions(out, exc, pos, &count);
 }
 
+// The reference's CPU decode of an ALPColumn (src/alp/alp-bindings.cu:208-228):
+// per vector, falp then patch_exceptions, for vectors [v0, v1); vector v goes
+// to out + v * 1024. A range lets a caller spread one column over threads.
+#define ALP_COLUMN_DECODE(T, UT, SUF)                                                              \
+  void ref_alp_column_decode_##SUF(size_t v0, size_t v1, const UT *packed,                         \
+                                   const uint8_t *bit_widths, const uint64_t *vector_offsets,     \
+                                   const UT *bases, const uint8_t *factor_indices,               \
+                                   const uint8_t *fraction_indices, const T *exceptions,         \
+                                   const uint16_t *positions, const uint64_t *exceptions_offsets, \
+                                   const uint16_t *counts, T *out) {                             \
+    for (size_t v = v0; v < v1; ++v) {                                                             \
+      generated::falp::fallback::scalar::falp(packed + vector_offsets[v], out + v * 1024,          \
+                                              bit_widths[v], &bases[v], factor_indices[v],         \
+                                              fraction_indices[v]);                                \
+      alp::decoder<T>::patch_exceptions(out + v * 1024, exceptions + exceptions_offsets[v],        \
+                                        positions + exceptions_offsets[v], &counts[v]);           \
+    }                                                                                              \
+  }
+ALP_COLUMN_DECODE(float, uint32_t, f32)
+ALP_COLUMN_DECODE(double, uint64_t, f64)
+
 size_t ref_alp_sample_f32(const float *data, size_t offset, size_t n, float *sample) {
   return alp::sampler::first_level_sample<float>(data, offset, n, sample);
 }

@@ -227,6 +227,31 @@ def test_alp_encode_matches_compiled_reference(seed):
 
 
 @needs_ref
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_reference_alp_column_decode_ranges(dtype):
+    """The compiled reference's column decode (falp + patch_exceptions per
+    vector, the CPU baseline's ALP leg) returns the encoded values, over the
+    whole column and over split vector ranges."""
+    from flsgpu import alp as AL
+    rng = np.random.default_rng(77)
+    n = 1024 * 37
+    if dtype == np.float64:
+        x = rng.integers(-10 ** 6, 10 ** 6, n) / 100.0
+    else:
+        x = rng.integers(-10 ** 4, 10 ** 4, n).astype(np.float32) / np.float32(10)
+    x[rng.choice(n, 400, replace=False)] = rng.uniform(-1e3, 1e3, 400).astype(dtype)
+    col = AL.alp_encode(x)
+    assert len(col.exceptions) > 0
+    whole = np.zeros(n, dtype=dtype)
+    O.ref_alp_column_decode_range(col, 0, col.bit_widths.size, whole)
+    assert np.array_equal(bits(whole), bits(x))
+    parts = np.full(n, np.nan, dtype=dtype)
+    for v0, v1 in ((0, 5), (5, 6), (6, 37)):
+        O.ref_alp_column_decode_range(col, v0, v1, parts)
+    assert np.array_equal(bits(parts), bits(x))
+
+
+@needs_ref
 def test_alp_encode_pieces_match_compiled_reference():
     rng = np.random.default_rng(7)
     for ft in (np.float32, np.float64):
