@@ -377,14 +377,14 @@ def filter_sum_extras(args, torch, A, D, E, dev, stream, r, flush):
 
 
 def selector_extras(torch, dev):
-    """Learned ALP selector: 2^18 training vectors (exhaustive-search labels
+    """Learned ALP selector: 2^24 training vectors (exhaustive-search labels
     on the host), trained on the GPU with the library's kernels; evaluated on
     a held-out 2^24-value double column (2^14 vectors): compression ratio and
     selection time vs the exhaustive search, pure proposal and verified
     top-4 proposal."""
     from flsgpu import selector as SEL
-    params, info = SEL.train_default(train_log2=18, epochs=4, seed=1, device=dev, log=None,
-                                     batch=1024)
+    params, info = SEL.train_default(train_log2=24, epochs=4, seed=1, device=dev, log=None,
+                                     batch=4096)
     res = SEL.evaluate(params, n_vecs=1 << 14, seed=99, device=dev, n_candidates=4)
     res.update({k: (round(v, 3) if isinstance(v, float) else v) for k, v in info.items()})
     return {k: (round(v, 6) if isinstance(v, float) else v) for k, v in res.items()}

@@ -326,8 +326,10 @@ int flsgpu_ffor_encode_device_u64(const uint64_t *in, uint64_t n_vecs, flsgpu_ff
 /* ---- batched mixed-encoding decode (no reference counterpart: the
  *      reference launches and synchronises once per column,
  *      src/engine/kernels.cuh:94-112). One persistent launch decodes every
- *      item from a device-side task queue; the queue re-arms itself, so the
- *      launch can be replayed from a hipGraph. --------------------------- */
+ *      item from a device-side task list that each wave walks on its own
+ *      (task t to wave t mod the grid's waves); no state carries over from
+ *      one launch to the next, so the launch can be replayed from a
+ *      hipGraph. ---------------------------------------------------------- */
 enum flsgpu_encoding {
   FLSGPU_ENC_BP_U32 = 0,    /* column: flsgpu_bp_column */
   FLSGPU_ENC_FFOR_U32 = 1,  /* column: flsgpu_ffor_column */
@@ -354,7 +356,8 @@ typedef struct flsgpu_batch_plan flsgpu_batch_plan;
 int flsgpu_batch_plan_create(const flsgpu_batch_item *items, uint32_t n_items, int variant,
                              flsgpu_batch_plan **plan);
 uint64_t flsgpu_batch_plan_tasks(const flsgpu_batch_plan *plan);
-/* One kernel launch for the whole batch. Launches of one plan must not overlap. */
+/* One kernel launch for the whole batch. Launches of one plan write the same
+ * outputs: they must not overlap. */
 int flsgpu_batch_run(flsgpu_batch_plan *plan, flsgpu_stream_t stream);
 /* Captures flsgpu_batch_run into a hipGraph on first use, then replays it. */
 int flsgpu_batch_graph_launch(flsgpu_batch_plan *plan, flsgpu_stream_t stream);

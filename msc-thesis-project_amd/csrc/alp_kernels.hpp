@@ -568,15 +568,14 @@ struct AlpStagePolicy {
 // each vector's exception list with coalesced loads and overwrites the
 // staged values whose positions fall in its slot range (positions are unique
 // in a vector, so the writes never collide), then 16-byte stores.
+// `lds`: the wave's stage, VPW * CHUNK elements, 16-byte aligned.
 template <typename T, int S, int LAYOUT, bool PATCH>
-__global__ __launch_bounds__(BLOCK) void k_alp_decompress_staged(AlpArgs<T> a, T *__restrict out) {
+__device__ __forceinline__ void alp_staged_unit(const AlpArgs<T> &a, T *__restrict out,
+                                                uint64_t wave, T *lds) {
   using UT = typename uint_of<T>::type;
   using L = lanes<UT>;
   using Sh = StageShape<UT, S>;
-  __shared__ __align__(16) T s_stage[BLOCK / WAVE][L::VPW * Sh::CHUNK];
-  const uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE;
   const UnitMap<UT, S> u = map_unit<UT, S>(wave);
-  T *lds = s_stage[threadIdx.x / WAVE];
   const int t = int(threadIdx.x % WAVE), vv = t / L::N;
   const int j0 = u.part * Sh::NJ;
   const uint64_t group = wave / S;
@@ -622,6 +621,14 @@ __global__ __launch_bounds__(BLOCK) void k_alp_decompress_staged(AlpArgs<T> a, T
   }
   wave_lds_sync();
   store_staged<T, S>(lds, out, wave / S, u.part, a.ffor.n_vecs);
+}
+
+template <typename T, int S, int LAYOUT, bool PATCH>
+__global__ __launch_bounds__(BLOCK) void k_alp_decompress_staged(AlpArgs<T> a, T *__restrict out) {
+  using UT = typename uint_of<T>::type;
+  __shared__ __align__(16) T s_stage[BLOCK / WAVE][lanes<UT>::VPW * StageShape<UT, S>::CHUNK];
+  alp_staged_unit<T, S, LAYOUT, PATCH>(a, out, (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE,
+                                       s_stage[threadIdx.x / WAVE]);
 }
 
 template <typename T, int S, bool NT, int LAYOUT, bool PATCH>

@@ -178,8 +178,8 @@ __device__ __forceinline__ bool dispatch_right_bw(Policy &pol, int bw) {
 // the decode made the time grow with the exception count (2^26 doubles: 201
 // us at 4 exceptions per vector, 289 us at 18); the LDS hand-over and the
 // register queue keep it flat.
-// Where lanes past a column's last vector write (see alprd_decompress_unit).
-// Never read.
+// Where lanes past a column's last vector write (see alprd_decompress_unit):
+// at most VPW - 1 vectors' lanes per launch. Never read.
 static __device__ uint64_t k_rd_sink[1024];
 
 // `list`: this wave's LDS, VPW * RD_LIST entries. The right-part words'
@@ -197,6 +197,9 @@ __device__ __forceinline__ void alprd_decompress_unit(const AlprdArgs<T> &a, T *
   using L = lanes<UT>;
   const UnitMap<UT, 1> u = map_unit<UT, 1>(wave);
   const int vv = int(threadIdx.x % WAVE) / L::N;
+  // a wave past the column's end leaves at once (all its lanes together);
+  // only the last wave can hold lanes on both sides of the end
+  if (uniform_u64(wave) * L::VPW >= a.n_vecs) return;
   const bool live = u.vec < a.n_vecs;
   const uint64_t vs = live ? u.vec : 0;
   const uint64_t eo = a.exceptions_offsets[vs];
@@ -319,19 +322,17 @@ __device__ __forceinline__ bool dispatch_right_bw_part(Policy &pol, int bw, int 
 // coalesced loads (issued before the decode) and rebuilds the values at the
 // positions in its slot range from the staged right part and the stored left
 // part; then 16-byte stores.
+// `dshift`: the pre-shifted dictionary in LDS; `lds`: the wave's stage,
+// VPW * CHUNK elements, 16-byte aligned.
 template <typename T, int S>
-__global__ __launch_bounds__(BLOCK) void k_alprd_decompress_staged(AlprdArgs<T> a,
-                                                                   T *__restrict out) {
+__device__ __forceinline__ void alprd_staged_unit(const AlprdArgs<T> &a, T *__restrict out,
+                                                  uint64_t wave,
+                                                  const typename uint_of<T>::type *dshift,
+                                                  T *lds) {
   using UT = typename uint_of<T>::type;
   using L = lanes<UT>;
   using Sh = StageShape<UT, S>;
-  __shared__ UT dshift[8];
-  __shared__ __align__(16) T s_stage[BLOCK / WAVE][L::VPW * Sh::CHUNK];
-  alprd_fill_dictionary<T>(a, dshift);
-  __syncthreads();
-  const uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE;
   const UnitMap<UT, S> u = map_unit<UT, S>(wave);
-  T *lds = s_stage[threadIdx.x / WAVE];
   const int t = int(threadIdx.x % WAVE), vv = t / L::N;
   const int j0 = u.part * Sh::NJ;
   const uint64_t group = wave / S;
@@ -395,6 +396,18 @@ __global__ __launch_bounds__(BLOCK) void k_alprd_decompress_staged(AlprdArgs<T> 
   }
   wave_lds_sync();
   store_staged<T, S>(lds, out, group, u.part, a.n_vecs);
+}
+
+template <typename T, int S>
+__global__ __launch_bounds__(BLOCK) void k_alprd_decompress_staged(AlprdArgs<T> a,
+                                                                   T *__restrict out) {
+  using UT = typename uint_of<T>::type;
+  __shared__ UT dshift[8];
+  __shared__ __align__(16) T s_stage[BLOCK / WAVE][lanes<UT>::VPW * StageShape<UT, S>::CHUNK];
+  alprd_fill_dictionary<T>(a, dshift);
+  __syncthreads();
+  alprd_staged_unit<T, S>(a, out, (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE, dshift,
+                          s_stage[threadIdx.x / WAVE]);
 }
 
 template <typename T, int S>

@@ -4,11 +4,18 @@
 // synchronises after each (src/engine/kernels.cuh:94-112,
 // src/engine/multi-column-host-kernels.cuh:31-115). Here a plan holds a device
 // table of column descriptors (FFOR/BP, ALP, G-ALP, ALP-RD at any width) and
-// a task list; a persistent grid pulls tasks (TASK_WAVES wave-units of one
-// column each) from a device-side atomic queue, so small and large columns of
-// different encodings share the chip. The queue re-arms itself when the last
-// block exits, so the single launch can be captured in a hipGraph and
-// replayed with no memset node.
+// a task list (TASK_UNITS wave-units of one column per task, column after
+// column). Every wave of a persistent grid walks the task list on its own
+// (task t goes to wave t mod the grid's waves), so consecutive tasks, which
+// belong to one column, run on different CUs and each wave sees a mix of
+// columns and encodings. No block barrier, atomic or queue is involved: a
+// wave's loop is bounded by the task count and nothing a wave does waits for
+// another wave, so one launch decodes the batch and can be captured in a
+// hipGraph as it is.
+//
+// Each encoding runs its LDS-staged decoder (decode into the wave's LDS
+// stage, exceptions patched there, 16-byte non-temporal stores) at a fixed
+// lane split: the shapes the per-column tuning picks for most columns.
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -20,19 +27,45 @@
 
 namespace flsgpu_hip {
 
-// Wave-units per task: each of the block's 4 waves runs one. Smaller tasks
-// balance the tail of a mixed batch better than the queue round trip costs
-// (C4, 96 columns: 16 per task 541 us, 8 522 us, 4 513 us).
-constexpr int TASK_WAVES = 4;
-#ifndef BATCH_NT
-#define BATCH_NT true
-#endif
+// Wave-units per task: consecutive units of one column (the parts of one
+// group of vectors, then the next group) run on one wave in a row.
+constexpr int TASK_UNITS = 4;
+
+// Lane split of the batch's wave-units, one per element width: every
+// encoding of that width shares one unpack body (see batch_unit).
+template <typename UT>
+constexpr int batch_split() {
+  return sizeof(UT) == 8 ? 4 : 2;
+}
+
+// LDS stage per wave: VPW * CHUNK elements of the larger shape (64-bit at
+// S = 4: 4 * 16 * 16 * 8 bytes; 32-bit at S = 2: 2 * 32 * 16 * 4 bytes).
+constexpr int BATCH_STAGE_WORDS = 1024;  // 64-bit words
+static_assert(lanes<uint64_t>::VPW * StageShape<uint64_t, batch_split<uint64_t>()>::CHUNK * 8 <=
+                  BATCH_STAGE_WORDS * 8 &&
+              lanes<uint32_t>::VPW * StageShape<uint32_t, batch_split<uint32_t>()>::CHUNK * 4 <=
+                  BATCH_STAGE_WORDS * 8,
+              "batch stage");
+
+enum BatchKind : uint32_t { BK_FFOR = 0, BK_ALP_STD = 1, BK_ALP_GALP = 2, BK_RD = 3 };
+
+template <typename UT>
+struct float_of;
+template <>
+struct float_of<uint32_t> {
+  using type = float;
+};
+template <>
+struct float_of<uint64_t> {
+  using type = double;
+};
 
 struct BatchColumn {
-  uint32_t enc;
-  uint32_t patch;
+  uint32_t enc;   // flsgpu_encoding
+  uint32_t kind;  // BatchKind
   void *out;
   uint64_t first_task;
+  uint64_t n_units;
   union {
     FforArgs<uint32_t> f32;
     FforArgs<uint64_t> f64;
@@ -43,88 +76,218 @@ struct BatchColumn {
   } u;
 };
 
-// Per-encoding task bodies are inlined into the kernel. As separate
-// (non-inlined) functions they follow the call convention, whose
-// callee-saved VGPRs sit in blocks of 8 across the whole register file: a
-// 64-bit body that needs ~130 caller-saved registers then touches v0..v247
-// and the kernel ran one wave per SIMD. Inlined, with a three-wave target,
-// the kernel takes 168 VGPRs (a few bytes of spill on the widest paths).
-template <typename UT, bool NT>
-__device__ __forceinline__ void batch_ffor(const BatchColumn *c, uint64_t w0, int wib) {
-  const FforArgs<UT> &a = *reinterpret_cast<const FforArgs<UT> *>(&c->u);
-  for (int k = 0; k < TASK_WAVES / 4; ++k)
-    ffor_decompress_unit<UT, 1, NT>(a, static_cast<UT *>(c->out), w0 + wib + 4 * k);
+// Makes the compiler wait for x at this point (an empty asm that reads it):
+// the wait for a loaded value is then placed before the loads that follow,
+// not where the value is first used, which may come after them (the load
+// counter drains in issue order, so waiting there would also wait for the
+// later loads).
+template <typename X>
+__device__ __forceinline__ void pin(const X &x) {
+#if defined(__HIP_DEVICE_COMPILE__)  // the device pass; a host build (tests/emu) has no VGPRs
+  asm volatile("" ::"v"(x));
+#else
+  (void)x;
+#endif
 }
 
-// G-ALP columns take the per-thread decoder with register patching (no LDS,
-// no barriers); standard-layout ones keep the per-thread scan of the list.
-template <typename T, int LAYOUT, bool NT>
-__device__ __forceinline__ void batch_alp(const BatchColumn *c, uint64_t w0, int wib) {
-  const AlpArgs<T> &a = *reinterpret_cast<const AlpArgs<T> *>(&c->u);
-  for (int k = 0; k < TASK_WAVES / 4; ++k) {
-    if constexpr (LAYOUT == ALP_GALP)
-      alp_decompress_unit<T, 1, NT, ALP_GALP, true>(a, static_cast<T *>(c->out), w0 + wib + 4 * k);
-    else
-      alp_decompress_unit_scan<T, NT, LAYOUT>(a, static_cast<T *>(c->out), w0 + wib + 4 * k);
+// One wave-unit (vector group w / S, lane part w % S) of a column of element
+// width UT, any encoding. The encodings share one unpack body: the packed
+// integers (plus the FOR base) are decoded into the wave's LDS stage, then
+// each thread turns its own cells into the encoding's values (ALP: the
+// integer-to-float formula; ALP-RD: the dictionary's left part joined to the
+// right part), the wave patches the exceptions it loaded before the decode,
+// and the stage is stored with 16-byte non-temporal stores. `dshift`: an
+// ALP-RD column's pre-shifted dictionary in LDS.
+template <typename UT>
+__device__ __forceinline__ void batch_unit(const BatchColumn &c, uint32_t kind, uint64_t w,
+                                           UT *stage, const UT *dshift) {
+  using T = typename float_of<UT>::type;
+  using ST = typename int_of<T>::type;
+  using L = lanes<UT>;
+  constexpr int S = batch_split<UT>();
+  using Sh = StageShape<UT, S>;
+  constexpr int VPW = L::VPW;
+  static_assert(offsetof(AlpArgs<T>, ffor) == 0, "ALP arguments start with the FFOR ones");
+  const FforArgs<UT> &f = *reinterpret_cast<const FforArgs<UT> *>(&c.u);
+  const AlpArgs<T> &al = *reinterpret_cast<const AlpArgs<T> *>(&c.u);
+  const AlprdArgs<T> &rd = *reinterpret_cast<const AlprdArgs<T> *>(&c.u);
+  const UnitMap<UT, S> u = map_unit<UT, S>(w);
+  const int t = int(threadIdx.x % WAVE), vv = t / L::N;
+  const int j0 = u.part * Sh::NJ;
+  const uint64_t g = uniform_u64(w / S);
+  const bool is_rd = kind == BK_RD, is_alp = kind == BK_ALP_STD || kind == BK_ALP_GALP;
+  const uint64_t n_vecs = is_rd ? rd.n_vecs : f.n_vecs;
+  const uint64_t vs = u.vec < n_vecs ? u.vec : 0;  // past the end: vector 0, never stored
+
+  // 1. metadata and exception list headers
+  const UT *in;
+  int bw;
+  UT base = 0;
+  unsigned fi = 0, ei = 0;
+  uint64_t eo[VPW];
+  unsigned raw[VPW];
+  LeftWords<UT> left;
+#pragma unroll
+  for (int v2 = 0; v2 < VPW; ++v2) {
+    eo[v2] = 0;
+    raw[v2] = 0;
   }
-}
-
-template <typename T, bool NT>
-__device__ __forceinline__ void batch_alprd(const BatchColumn *c, uint64_t w0, int wib,
-                                         const typename uint_of<T>::type *dict, uint32_t *list) {
-  const AlprdArgs<T> &a = *reinterpret_cast<const AlprdArgs<T> *>(&c->u);
-  for (int k = 0; k < TASK_WAVES / 4; ++k)
-    alprd_decompress_unit<T, NT>(a, static_cast<T *>(c->out), w0 + wib + 4 * k, dict, list);
-}
-
-template <bool NT>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3))) void k_batch(
-    const BatchColumn *__restrict cols, const uint32_t *__restrict task_col, uint64_t n_tasks,
-    unsigned long long *queue) {
-  __shared__ unsigned long long s_task;
-  __shared__ uint64_t dict_lds[8];
-  static_assert(lanes<uint32_t>::VPW <= lanes<uint64_t>::VPW, "rd_list sized for the widest VPW");
-  __shared__ uint32_t rd_list[BLOCK / WAVE][lanes<uint64_t>::VPW * RD_LIST];
-  const int wib = int(threadIdx.x / WAVE);
-  for (;;) {
-    __syncthreads();  // every wave is done with the previous task (and its LDS)
-    if (threadIdx.x == 0) s_task = atomicAdd(&queue[0], 1ull);
-    __syncthreads();
-    const uint64_t t = s_task;
-    if (t >= n_tasks) break;
-    const BatchColumn *c = cols + task_col[t];
-    const uint64_t w0 = (t - c->first_task) * TASK_WAVES;
-    switch (c->enc) {
-      case FLSGPU_ENC_BP_U32:
-      case FLSGPU_ENC_FFOR_U32: batch_ffor<uint32_t, NT>(c, w0, wib); break;
-      case FLSGPU_ENC_BP_U64:
-      case FLSGPU_ENC_FFOR_U64: batch_ffor<uint64_t, NT>(c, w0, wib); break;
-      case FLSGPU_ENC_ALP_F32: batch_alp<float, ALP_STD, NT>(c, w0, wib); break;
-      case FLSGPU_ENC_ALP_EXT_F32: batch_alp<float, ALP_GALP, NT>(c, w0, wib); break;
-      case FLSGPU_ENC_ALP_F64: batch_alp<double, ALP_STD, NT>(c, w0, wib); break;
-      case FLSGPU_ENC_ALP_EXT_F64: batch_alp<double, ALP_GALP, NT>(c, w0, wib); break;
-      case FLSGPU_ENC_ALPRD_F32: {
-        uint32_t *d = reinterpret_cast<uint32_t *>(dict_lds);
-        alprd_fill_dictionary<float>(c->u.r32, d);
-        __syncthreads();
-        batch_alprd<float, NT>(c, w0, wib, d, rd_list[wib]);
-        break;
+  if (is_rd) {
+    in = rd.right + vs * uint64_t(rd.right_bw) * L::N + u.lane;
+    bw = int(rd.right_bw);
+    left.load(rd.left + vs * uint64_t(rd.left_bw) * 64, u.lane, rd.left_bw);
+#pragma unroll
+    for (int v2 = 0; v2 < VPW; ++v2) {
+      const uint64_t vec = g * VPW + v2, s2 = vec < n_vecs ? vec : 0;
+      eo[v2] = rd.exceptions_offsets[s2];
+      raw[v2] = rd.counts[s2];
+    }
+  } else {
+    in = f.packed + f.vector_offsets[vs] + u.lane;
+    bw = f.bit_widths[vs];
+    base = f.bases ? f.bases[vs] : UT(0);
+    if (is_alp) {
+      fi = al.factor_indices[vs];
+      ei = al.fraction_indices[vs];
+#pragma unroll
+      for (int v2 = 0; v2 < VPW; ++v2) {
+        const uint64_t vec = g * VPW + v2, s2 = vec < n_vecs ? vec : 0;
+        eo[v2] = al.exceptions_offsets[s2];
+        raw[v2] = kind == BK_ALP_GALP ? unsigned(al.offsets_counts[s2 * L::N + (L::N - 1)])
+                                      : unsigned(al.counts[s2]);
       }
-      case FLSGPU_ENC_ALPRD_F64:
-        alprd_fill_dictionary<double>(c->u.r64, dict_lds);
-        __syncthreads();
-        batch_alprd<double, NT>(c, w0, wib, dict_lds, rd_list[wib]);
-        break;
-      default:
-        break;
     }
   }
-  // Re-arm the queue: the last block to leave resets both counters.
-  if (threadIdx.x == 0) {
-    __threadfence();
-    if (atomicAdd(&queue[1], 1ull) == gridDim.x - 1) {
-      __hip_atomic_store(&queue[0], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&queue[1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned count[VPW];
+#pragma unroll
+  for (int v2 = 0; v2 < VPW; ++v2) {
+    // G-ALP lanes are concatenated in order: the last lane's offset + count
+    const unsigned cnt = kind == BK_ALP_GALP ? (raw[v2] & 0x3FFu) + (raw[v2] >> 10) : raw[v2];
+    count[v2] = __builtin_amdgcn_readfirstlane(g * VPW + v2 < n_vecs ? cnt : 0u);
+  }
+  pin(in);
+  pin(bw);
+  pin(base);
+
+  // 2. the lists' first 64 entries (entry t to thread t), then the packed
+  // words: one round trip covers both
+  unsigned ppos[VPW];
+  UT pval[VPW];
+  if (is_rd) {
+#pragma unroll
+    for (int v2 = 0; v2 < VPW; ++v2) {
+      ppos[v2] = *list_entry(rd.positions, eo[v2], unsigned(t), count[v2]);
+      pval[v2] = UT(*list_entry(rd.exceptions, eo[v2], unsigned(t), count[v2]));
+    }
+  } else {
+    const UT *exc = reinterpret_cast<const UT *>(al.exceptions);
+#pragma unroll
+    for (int v2 = 0; v2 < VPW; ++v2) {
+      ppos[v2] = *list_entry(al.positions, eo[v2], unsigned(t), count[v2]);
+      pval[v2] = *list_entry(exc, eo[v2], unsigned(t), count[v2]);
+    }
+  }
+  UT *row = stage + vv * Sh::CHUNK + u.lane;
+  StagePolicy<UT> pol{in, row, base};
+  if (!dispatch_bw<UT, S>(pol, bw, u.part)) {
+    for (int jj = 0; jj < Sh::NJ; ++jj) row[jj * L::N] = UT(0);
+  }
+
+  // 3. the thread's own cells to the encoding's values
+  if (is_alp) {
+    const UT fact = alp_factor<T>(fi);
+    const T frac = alp_fraction<T>(ei);
+#pragma unroll
+    for (int jj = 0; jj < Sh::NJ; ++jj) {
+      const UT x = row[jj * L::N];
+      row[jj * L::N] = __builtin_bit_cast(UT, T(ST(UT(x * fact))) * frac);
+    }
+  } else if (is_rd) {
+    constexpr int Q = 64 / L::N;
+    uint64_t lstream[Q];
+    left.compose(rd.left_bw, lstream);
+    const unsigned lmask = (1u << rd.left_bw) - 1u;
+#pragma unroll
+    for (int jj = 0; jj < Sh::NJ; ++jj) {
+      const int j = j0 + jj;
+      uint64_t ls = lstream[0];
+#pragma unroll
+      for (int q = 1; q < Q; ++q) ls = (j % Q) == q ? lstream[q] : ls;
+      const unsigned idx = unsigned(ls >> ((j / Q) * rd.left_bw)) & lmask;
+      row[jj * L::N] = UT(dshift[idx] | row[jj * L::N]);
+    }
+  }
+
+  // 4. exceptions: each position is written once (positions are unique in a
+  // vector), by the thread that loaded it
+  if (is_alp || is_rd) {
+    wave_lds_sync();
+    const unsigned lo = unsigned(L::N * j0), hi = unsigned(L::N * (j0 + Sh::NJ));
+    const UT rmask = is_rd ? UT((UT(1) << rd.right_bw) - 1) : UT(0);
+    const unsigned rbw = is_rd ? rd.right_bw : 0u;
+    const uint16_t *positions = is_rd ? rd.positions : al.positions;
+#pragma unroll
+    for (int v2 = 0; v2 < VPW; ++v2) {
+      const unsigned pos = ppos[v2] & 1023u;
+      if (unsigned(t) < count[v2] && pos >= lo && pos < hi) {
+        UT &x = stage[v2 * Sh::CHUNK + (pos - lo)];
+        x = is_rd ? UT(UT(pval[v2] << rbw) | (x & rmask)) : pval[v2];
+      }
+      for (unsigned k = unsigned(t + WAVE); k < count[v2]; k += WAVE) {
+        const unsigned p2 = positions[eo[v2] + k] & 1023u;
+        if (p2 >= lo && p2 < hi) {
+          UT &x = stage[v2 * Sh::CHUNK + (p2 - lo)];
+          x = is_rd ? UT(UT(UT(rd.exceptions[eo[v2] + k]) << rbw) | (x & rmask))
+                    : reinterpret_cast<const UT *>(al.exceptions)[eo[v2] + k];
+        }
+      }
+    }
+  }
+  wave_lds_sync();
+  store_staged<UT, S>(stage, static_cast<UT *>(c.out), g, u.part, n_vecs);
+}
+
+template <typename UT>
+__device__ __forceinline__ void batch_task(const BatchColumn &c, uint64_t u0, uint64_t u1,
+                                           UT *stage, UT *dict) {
+  using T = typename float_of<UT>::type;
+  const uint32_t kind = c.kind;
+  if (kind == BK_RD) {  // this column's dictionary, pre-shifted, in the wave's LDS
+    const AlprdArgs<T> &rd = *reinterpret_cast<const AlprdArgs<T> *>(&c.u);
+    const unsigned lane = threadIdx.x % WAVE;
+    wave_lds_sync();  // the previous task's dictionary reads are done
+    if (lane < 8) dict[lane] = lane < rd.dict_size ? UT(UT(rd.dict[lane]) << rd.right_bw) : UT(0);
+  }
+  for (uint64_t w = u0; w < u1; ++w) {
+    wave_lds_sync();  // the previous unit's stage reads (and the dictionary writes) are done
+    batch_unit<UT>(c, kind, w, stage, dict);
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_batch(const BatchColumn *__restrict cols,
+                                                 const uint32_t *__restrict task_col,
+                                                 uint64_t n_tasks) {
+  __shared__ __align__(16) uint64_t s_stage[BLOCK / WAVE][BATCH_STAGE_WORDS];
+  __shared__ uint64_t s_dict[BLOCK / WAVE][8];
+  const int wib = int(threadIdx.x / WAVE);
+  const uint64_t stride = uint64_t(gridDim.x) * (BLOCK / WAVE);
+  for (uint64_t t = uniform_u64(uint64_t(blockIdx.x) * (BLOCK / WAVE) + wib); t < n_tasks;
+       t += stride) {
+    const BatchColumn &c = cols[task_col[t]];
+    const uint64_t u0 = (t - c.first_task) * TASK_UNITS;
+    const uint64_t u1 = u0 + TASK_UNITS < c.n_units ? u0 + TASK_UNITS : c.n_units;
+    switch (c.enc) {
+      case FLSGPU_ENC_BP_U32:
+      case FLSGPU_ENC_FFOR_U32:
+      case FLSGPU_ENC_ALP_F32:
+      case FLSGPU_ENC_ALP_EXT_F32:
+      case FLSGPU_ENC_ALPRD_F32:
+        batch_task<uint32_t>(c, u0, u1, reinterpret_cast<uint32_t *>(s_stage[wib]),
+                             reinterpret_cast<uint32_t *>(s_dict[wib]));
+        break;
+      default:
+        batch_task<uint64_t>(c, u0, u1, s_stage[wib], s_dict[wib]);
+        break;
     }
   }
 }
@@ -136,10 +299,8 @@ using namespace flsgpu_hip;
 struct flsgpu_batch_plan {
   BatchColumn *d_cols = nullptr;
   uint32_t *d_task_col = nullptr;
-  unsigned long long *d_queue = nullptr;
   uint64_t n_tasks = 0;
   unsigned grid = 1;
-  bool nt = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t graph_stream = nullptr;
@@ -147,14 +308,28 @@ struct flsgpu_batch_plan {
 
 namespace {
 
-uint64_t waves_of(uint32_t enc, uint64_t n_vecs) {
+// Wave-units of a column in the batch (its vectors' groups times the lane
+// split).
+uint64_t units_of(uint32_t enc, uint64_t n_vecs) {
   switch (enc) {
     case FLSGPU_ENC_BP_U32:
     case FLSGPU_ENC_FFOR_U32:
     case FLSGPU_ENC_ALP_F32:
     case FLSGPU_ENC_ALP_EXT_F32:
-    case FLSGPU_ENC_ALPRD_F32: return n_waves_for<uint32_t, 1>(n_vecs);
-    default: return n_waves_for<uint64_t, 1>(n_vecs);
+    case FLSGPU_ENC_ALPRD_F32: return n_waves_for<uint32_t, batch_split<uint32_t>()>(n_vecs);
+    default: return n_waves_for<uint64_t, batch_split<uint64_t>()>(n_vecs);
+  }
+}
+
+uint32_t kind_of(uint32_t enc) {
+  switch (enc) {
+    case FLSGPU_ENC_ALP_F32:
+    case FLSGPU_ENC_ALP_F64: return BK_ALP_STD;
+    case FLSGPU_ENC_ALP_EXT_F32:
+    case FLSGPU_ENC_ALP_EXT_F64: return BK_ALP_GALP;
+    case FLSGPU_ENC_ALPRD_F32:
+    case FLSGPU_ENC_ALPRD_F64: return BK_RD;
+    default: return BK_FFOR;
   }
 }
 
@@ -165,7 +340,6 @@ void destroy(flsgpu_batch_plan *p) {
   if (p->graph_stream) (void)hipStreamDestroy(p->graph_stream);
   (void)hipFree(p->d_cols);
   (void)hipFree(p->d_task_col);
-  (void)hipFree(p->d_queue);
   delete p;
 }
 
@@ -243,29 +417,38 @@ int flsgpu_batch_plan_create(const flsgpu_batch_item *items, uint32_t n_items, i
     if (is_alp && (it.patcher == FLSGPU_PATCHER_NONE || it.patcher == FLSGPU_PATCHER_DUMMY))
       return set_error(FLSGPU_EUNSUPPORTED,
                        "batch item %u: the batch decoder always patches exceptions", i);
-    c.patch = is_alp;
-    const uint64_t tasks = (waves_of(c.enc, n_vecs) + TASK_WAVES - 1) / TASK_WAVES;
+    c.kind = kind_of(c.enc);
+    c.n_units = units_of(c.enc, n_vecs);
+    const uint64_t tasks = (c.n_units + TASK_UNITS - 1) / TASK_UNITS;
     for (uint64_t k = 0; k < tasks; ++k) task_col.push_back(i);
   }
   auto *p = new (std::nothrow) flsgpu_batch_plan();
   if (!p) return set_error(FLSGPU_ENOMEM, "out of host memory");
-  (void)variant;  // one store flavour is compiled into the batch kernel (BATCH_NT)
-  p->nt = BATCH_NT;
+  // every encoding runs its one batch shape (batch_split); `variant` selects
+  // nothing here
+  (void)variant;
   p->n_tasks = task_col.size();
   int rc;
   if ((rc = check_hip(hipMalloc(&p->d_cols, sizeof(BatchColumn) * (cols.size() + 1)), "hipMalloc")) ||
       (rc = check_hip(hipMalloc(&p->d_task_col, sizeof(uint32_t) * (task_col.size() + 1)), "hipMalloc")) ||
-      (rc = check_hip(hipMalloc(&p->d_queue, 2 * sizeof(unsigned long long)), "hipMalloc")) ||
       (rc = check_hip(hipMemcpy(p->d_cols, cols.data(), sizeof(BatchColumn) * cols.size(),
                                 hipMemcpyHostToDevice), "hipMemcpy")) ||
       (rc = check_hip(hipMemcpy(p->d_task_col, task_col.data(), sizeof(uint32_t) * task_col.size(),
-                                hipMemcpyHostToDevice), "hipMemcpy")) ||
-      (rc = check_hip(hipMemset(p->d_queue, 0, 2 * sizeof(unsigned long long)), "hipMemset"))) {
+                                hipMemcpyHostToDevice), "hipMemcpy"))) {
     destroy(p);
     return rc;
   }
-  const uint64_t max_blocks = uint64_t(num_cus()) * 8;
-  p->grid = unsigned(p->n_tasks < max_blocks ? (p->n_tasks ? p->n_tasks : 1) : max_blocks);
+  // as many blocks as fit on the device at once (every wave then walks
+  // its share of the task list), fewer for a small batch
+  int per_cu = 0;
+  if ((rc = check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch, BLOCK, 0),
+                      "hipOccupancyMaxActiveBlocksPerMultiprocessor"))) {
+    destroy(p);
+    return rc;
+  }
+  const uint64_t max_blocks = uint64_t(num_cus()) * uint64_t(per_cu > 0 ? per_cu : 1);
+  const uint64_t want = (p->n_tasks + (BLOCK / WAVE) - 1) / (BLOCK / WAVE);
+  p->grid = unsigned(want < max_blocks ? (want ? want : 1) : max_blocks);
   *plan_out = p;
   return FLSGPU_OK;
 }
@@ -275,8 +458,8 @@ uint64_t flsgpu_batch_plan_tasks(const flsgpu_batch_plan *plan) { return plan ? 
 int flsgpu_batch_run(flsgpu_batch_plan *plan, flsgpu_stream_t stream) {
   if (!plan) return set_error(FLSGPU_EINVAL, "null plan");
   if (plan->n_tasks == 0) return FLSGPU_OK;
-  hipLaunchKernelGGL((k_batch<BATCH_NT>), dim3(plan->grid), dim3(BLOCK), 0, stream, plan->d_cols,
-                     plan->d_task_col, plan->n_tasks, plan->d_queue);
+  hipLaunchKernelGGL(k_batch, dim3(plan->grid), dim3(BLOCK), 0, stream, plan->d_cols,
+                     plan->d_task_col, plan->n_tasks);
   return check_hip(hipGetLastError(), "k_batch launch");
 }
 

@@ -237,17 +237,14 @@ __global__ __launch_bounds__(BLOCK) void k_ffor_decompress(FforArgs<UT> a, UT *_
   ffor_decompress_unit<UT, S, NT>(a, out, (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE);
 }
 
-// Staged variant: one wave-unit per wave, decoded into LDS, stored 16 bytes
-// per lane.
+// Staged variant: one wave-unit, decoded into the wave's LDS stage `lds`
+// (VPW * CHUNK elements, 16-byte aligned), stored 16 bytes per lane.
 template <typename UT, int S>
-__global__ __launch_bounds__(BLOCK) void k_ffor_decompress_staged(FforArgs<UT> a,
-                                                                 UT *__restrict out) {
+__device__ __forceinline__ void ffor_staged_unit(const FforArgs<UT> &a, UT *__restrict out,
+                                                 uint64_t wave, UT *lds) {
   using L = lanes<UT>;
   using Sh = StageShape<UT, S>;
-  __shared__ __align__(16) UT s_stage[BLOCK / WAVE][L::VPW * Sh::CHUNK];
-  const uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE;
   const UnitMap<UT, S> u = map_unit<UT, S>(wave);
-  UT *lds = s_stage[threadIdx.x / WAVE];
   const int vv = int(threadIdx.x % WAVE) / L::N;
   if (u.vec < a.n_vecs) {
     StagePolicy<UT> pol{a.packed + a.vector_offsets[u.vec] + u.lane, lds + vv * Sh::CHUNK + u.lane,
@@ -258,6 +255,14 @@ __global__ __launch_bounds__(BLOCK) void k_ffor_decompress_staged(FforArgs<UT> a
   }
   wave_lds_sync();
   store_staged<UT, S>(lds, out, wave / S, u.part, a.n_vecs);
+}
+
+template <typename UT, int S>
+__global__ __launch_bounds__(BLOCK) void k_ffor_decompress_staged(FforArgs<UT> a,
+                                                                 UT *__restrict out) {
+  __shared__ __align__(16) UT s_stage[BLOCK / WAVE][lanes<UT>::VPW * StageShape<UT, S>::CHUNK];
+  ffor_staged_unit<UT, S>(a, out, (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE,
+                          s_stage[threadIdx.x / WAVE]);
 }
 
 // Blocks of the scanning kernels (query any, fused filter + sum): enough to

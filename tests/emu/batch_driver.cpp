@@ -1,7 +1,7 @@
-// Runs the batched decoder (csrc/batch.hip: plan, device queue, persistent
+// Runs the batched decoder (csrc/batch.hip: plan, task list, persistent
 // kernel) on the host emulation: reads the columns of a batch from a file
-// into exact-size buffers, creates the plan, runs it twice (the queue re-arms
-// itself) and compares every output with the expected values.
+// into exact-size buffers, creates the plan, runs it twice (a plan is
+// reusable) and compares every output with the expected values.
 //   batch_driver <batch>    EMU_BALLOT=0|1 selects the unpack path
 //   batch: uint32 count; per column int32 encoding, uint64 n_vecs, int32
 //   patcher, the descriptor's arrays (uint64 byte count + bytes each; ALP-RD

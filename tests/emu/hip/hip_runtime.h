@@ -7,8 +7,7 @@
 // threads (every shuffle in these kernels stays inside one vector's
 // N_LANES-thread group), readfirstlane is the identity and the ballot
 // answers with a fixed mode: 0 sends every lane down the uniform-width path
-// with its own width, 1 down the generic mixed-width path. A persistent grid
-// drains its queue in the first block.
+// with its own width, 1 down the generic mixed-width path.
 #pragma once
 #include <barrier>
 #include <climits>
@@ -130,9 +129,14 @@ inline void emu_launch(K kernel, dim3 grid, dim3 block, A... args) {
   }
 }
 #define hipLaunchKernelGGL(k, g, b, sh, st, ...) emu_launch(k, dim3(g), dim3(b), __VA_ARGS__)
+template <class K>
+inline hipError_t hipOccupancyMaxActiveBlocksPerMultiprocessor(int *n, K, int, size_t) {
+  *n = 2;
+  return hipSuccess;
+}
 
-// The batch kernel's plan and queue: device memory is host memory here,
-// global atomics are host atomics, graph capture is unavailable.
+// The batch kernel's plan: device memory is host memory here, graph capture
+// is unavailable.
 enum hipMemcpyKind { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2 };
 template <class T>
 inline hipError_t hipMalloc(T **p, size_t n) {

@@ -1,7 +1,7 @@
 """GPU parity of the batched persistent decoder: every encoding in one
 launch, bit for bit equal to the CPU oracle's decode of the same encoded
-buffers and to the original values; the self-re-arming queue re-runs
-correctly, plainly and from hipGraph replays. The full-size case is the
+buffers and to the original values; a plan re-runs correctly, plainly and
+from hipGraph replays. The full-size case is the
 benchmark's own 96-column mixed batch (2^12..2^24 values per column)."""
 import numpy as np
 import pytest
@@ -95,7 +95,7 @@ def test_batch_every_encoding_matches_oracle():
     plan.run()
     torch.cuda.synchronize()
     check_all(specs, expected, outs, "run 1")
-    # the queue re-armed itself: a second run decodes everything again
+    # a plan is reusable: a second run decodes everything again
     for o in outs:
         o.zero_()
     plan.run()

@@ -1,6 +1,6 @@
 """The device encoder's kernels (csrc/encode.hip), the ALP decoders
-(csrc/alp_kernels.hpp) and the batched decoder (csrc/batch.hip: plan, queue,
-persistent kernel over every encoding) run on the CPU through a host emulation of the HIP
+(csrc/alp_kernels.hpp) and the batched decoder (csrc/batch.hip: plan, task
+list, persistent kernel over every encoding) run on the CPU through a host emulation of the HIP
 features they use (tests/emu/hip/hip_runtime.h: one OS thread per GPU
 thread, block and wave barriers, group shuffles), built with
 AddressSanitizer. Every output buffer has the exact size the library's
