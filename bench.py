@@ -343,7 +343,8 @@ def c1_extras(args, torch, A, D, E, dev, stream, r, flush, host=None):
     algo = p.nbytes + b.nbytes + o.nbytes + s.nbytes + nbytes
     res["c1_u32_w13_2^25_decode"] = {"us": round(ms * 1e3, 2),
                                       "decompressed_GBps": round(nbytes / ms / 1e6, 1),
-                                      "hbm_frac": round(algo / ms / 1e6 / HBM_PEAK_GBS, 4)}
+                                      "hbm_frac": round(algo / ms / 1e6 / HBM_PEAK_GBS, 4),
+                                      "algorithmic_bytes": int(algo)}
 
     ms = cold_launch_ms(torch, stream, lambda: D.query_filter_sum(col, A.PRED_ALL, result=r,
                                                                   stream=stream), 20, flush,
@@ -352,6 +353,7 @@ def c1_extras(args, torch, A, D, E, dev, stream, r, flush, host=None):
     res["c1_u32_w13_2^25_sum"] = {"us": round(ms * 1e3, 2),
                                    "decompressed_equiv_GBps": round(nbytes / ms / 1e6, 1),
                                    "hbm_frac": round((algo - nbytes) / ms / 1e6 / HBM_PEAK_GBS, 4),
+                                   "algorithmic_bytes": int(algo - nbytes),
                                    "sum_ok": got_sum == int(vals.astype(np.uint64).sum())}
     del col
     return res
@@ -497,6 +499,7 @@ def alp_extras(torch, dev, stream, flush, host=None):
             res[f"c3_{name}_f{width}_2^{int(np.log2(len(x)))}"] = {
                 "us": round(ms * 1e3, 1), "decompressed_GBps": round(nbytes / ms / 1e6, 1),
                 "hbm_frac": round((read + nbytes) / ms / 1e6 / HBM_PEAK_GBS, 4), "bit_exact": ok,
+                "algorithmic_bytes": int(read + nbytes),
                 "bits_per_value": round(float(col.bit_widths.mean()), 2),
                 "exceptions_per_vector": round(len(col.exceptions) / col.n_vecs, 2),
                 "compression_ratio": round(c.compression_ratio(extended=name == "galp"), 3),
@@ -526,6 +529,7 @@ def alp_extras(torch, dev, stream, flush, host=None):
                                   rd.exceptions, rd.positions, rd.counts))
     res["alprd_f64_2^26"] = {"us": round(ms * 1e3, 1), "decompressed_GBps": round(x.nbytes / ms / 1e6, 1),
                              "hbm_frac": round((read + x.nbytes) / ms / 1e6 / HBM_PEAK_GBS, 4),
+                             "algorithmic_bytes": int(read + x.nbytes),
                              "bit_exact": ok, "right_bit_width": int(rd.right_bw),
                              "left_bit_width": int(rd.left_bw),
                              "exceptions_per_vector": round(len(rd.exceptions) / rd.n_vecs, 2),
@@ -616,8 +620,13 @@ def cpu_baseline(args, host):
         nbytes = sum(b for _, _, b in items)
         t1 = _median_s(lambda: [fn(0, n) for fn, n, _ in items])
 
+        # pieces of about an equal share of all vectors (many small columns
+        # go whole to one thread; a large one is split)
+        piece = max(1, sum(n for _, n, _ in items) // (4 * n_threads))
+
         def par():
-            futs = [pool.submit(fn, a, b) for fn, n, _ in items for a, b in _ranges(n, n_threads)]
+            futs = [pool.submit(fn, a, b) for fn, n, _ in items
+                    for a, b in _ranges(n, max(1, -(-n // piece)))]
             for f in futs:
                 f.result()
         tn = _median_s(par)
@@ -665,7 +674,8 @@ def cpu_baseline(args, host):
                       f"copied from the device, fls::unffor per vector, median of {CPU_RUNS} "
                       f"runs, single thread",
             "all_cores": {"value": many, "unit": "GB/s", "cores": n_threads,
-                          "sample": f"the same columns, each split over {n_threads} threads"},
+                          "sample": f"the same columns over {n_threads} threads, in pieces of "
+                                    f"about 1/{4 * n_threads} of the vectors"},
             "configs": configs, "cpu_seconds": round(time.perf_counter() - t0, 1),
             "cpu_model": model, "host_cpus": os.cpu_count(),
             "affinity_cpus": len(os.sched_getaffinity(0))}
