@@ -69,79 +69,14 @@ struct AnyPolicy {
   }
 };
 
-// Fused filter + sum. With no wrap-around in (offset + base) the sum of the
-// passing values is sum(offsets) + count * base, so the per-value work is an
-// unsigned range test and one narrow add on the FOR offset; the base is
-// applied once per lane (the ALU cost per value, not HBM, otherwise bounds
-// this kernel). Columns whose base + 2^BW - 1 wraps take the per-value path.
+// Fused filter + sum, runtime-width path (wavefronts whose vectors differ in
+// bit width): exact per value.
 template <typename UT, int PRED>
 struct FilterSumPolicy {
   const UT *in;
   UT base;
   UT lo, hi;
   uint64_t count, sum;
-  template <int BW, int J0, int NJ>
-  __device__ __forceinline__ void run() {
-    constexpr int W = lanes<UT>::W;
-    constexpr UT MAXOFF = BW == W ? UT(~UT(0)) : UT((UT(1) << BW) - 1);
-    // per-lane offset sums stay below 2^32 when BW + log2(NJ) <= 32
-    using Acc = std::conditional_t<(W == 32 && BW + 6 <= 32), uint32_t, uint64_t>;
-    const bool no_wrap = base <= UT(UT(~UT(0)) - MAXOFF);
-    if constexpr (PRED == FLSGPU_PRED_OFFSET_LOWER_HALF) {
-      if constexpr (BW > 0) {  // no row passes at bit width 0
-        if (W == 64 || no_wrap) {  // 64-bit sums are mod 2^64 anyway
-          Acc acc = 0;
-          uint32_t cnt = 0;
-          unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT off) {
-            const bool pass = (off >> (BW - 1)) == 0;
-            cnt += pass;
-            acc += pass ? Acc(off) : Acc(0);
-          });
-          count += cnt;
-          sum += uint64_t(acc) + uint64_t(cnt) * uint64_t(base);
-        } else {
-          unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT off) {
-            const bool pass = (off >> (BW - 1)) == 0;
-            count += pass;
-            sum += pass ? uint64_t(UT(off + base)) : 0;
-          });
-        }
-      }
-    } else if constexpr (PRED == FLSGPU_PRED_ALL) {
-      if (W == 64 || no_wrap) {
-        Acc acc = 0;
-        unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT off) { acc += Acc(off); });
-        count += NJ;
-        sum += uint64_t(acc) + uint64_t(NJ) * uint64_t(base);
-      } else {
-        unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT off) { sum += uint64_t(UT(off + base)); });
-        count += NJ;
-      }
-    } else {
-      if (no_wrap) {
-        // lo <= base + off < hi  <=>  off - olo < span (unsigned)
-        const UT from = lo > base ? lo : base;
-        const UT olo = UT(from - base);
-        const UT span = hi > from ? UT(hi - from) : UT(0);
-        Acc acc = 0;
-        uint32_t cnt = 0;
-        unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT off) {
-          const bool pass = UT(off - olo) < span;
-          cnt += pass;
-          acc += pass ? Acc(off) : Acc(0);
-        });
-        count += cnt;
-        sum += uint64_t(acc) + uint64_t(cnt) * uint64_t(base);
-      } else {
-        unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT off) {
-          const UT v = UT(off + base);
-          const bool pass = (v >= lo) & (v < hi);
-          count += pass;
-          sum += pass ? uint64_t(v) : 0;
-        });
-      }
-    }
-  }
   __device__ __forceinline__ void run_generic(int bw, int j0, int nj) {
     const UT half = bw > 0 ? UT(UT(1) << (bw - 1)) : UT(0);
     unpack_runtime<UT>(in, bw, j0, nj, [&](int, UT off) {
@@ -155,6 +90,190 @@ struct FilterSumPolicy {
     });
   }
 };
+
+// Bit (q mod 32) of word q / 32 of a span's 32-bit word stream is set for
+// every slot j in [J0, J0 + NJ) whose bit B sits at span position
+// q = j*BW + B - 32*H0 (H0 = the span's first 32-bit lane word).
+template <int BW, int J0, int NJ, int B, int H0>
+__host__ __device__ constexpr uint32_t slot_bit_mask(int word) {
+  uint32_t m = 0;
+  for (int j = J0; j < J0 + NJ; ++j) {
+    const int q = j * BW + B - 32 * H0;
+    if (q >= 32 * word && q < 32 * word + 32) m |= 1u << (q - 32 * word);
+  }
+  return m;
+}
+
+__host__ __device__ constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
+
+// Count of slots [J0, J0 + NJ) of one lane that pass PRED (OFFSET_LOWER_HALF:
+// the FOR offset's top bit is clear; ALL) and the sum of their offsets mod
+// 2^64, from the span's words in registers. The work is on the packed words,
+// not per value where a bit trick allows:
+//  - the passing count is NJ minus a popcount of the top-bit positions;
+//  - OFFSET_LOWER_HALF at BW <= 4 (ALL at BW <= 2): the offset sum is a
+//    popcount per bit plane b (bit b of every passing field, weighted 2^b);
+//    bit b of a field passes when the top bit, BW-1-b positions higher, is
+//    clear (one funnel shift of the stream);
+//  - otherwise one signed field extract per value (v_bfe_i32, or a funnel
+//    shift first for a field across two words): a passing offset is the
+//    non-negative sign-extended field, so max(field, 0) is the term.
+// Fields below 32 bits are read from the lane's stream as 32-bit words (for
+// 64-bit types the low half of each word first), so they span at most two.
+template <typename UT, int BW, int J0, int NJ, int PRED>
+__device__ __forceinline__ void span_offset_stats(const UT *w, uint32_t &cnt, uint64_t &offsum) {
+  constexpr int W = lanes<UT>::W;
+  using Sp = SlotSpan<UT, BW, J0, NJ>;
+  constexpr int H0 = Sp::R0 * (W / 32);
+  constexpr int H = Sp::NR * (W / 32);
+  auto hw = [&](auto ic) -> uint32_t {
+    constexpr int i = decltype(ic)::value;
+    if constexpr (i >= H) return 0u;
+    else if constexpr (W == 32) return uint32_t(w[i]);
+    else return uint32_t(uint64_t(w[i / 2]) >> (32 * (i % 2)));
+  };
+  constexpr bool LOWER = PRED == FLSGPU_PRED_OFFSET_LOWER_HALF;
+  if constexpr (BW == 0) {
+    cnt = LOWER ? 0 : NJ;  // no offset is below 2^-1
+    offsum = 0;
+    return;
+  } else {
+    if constexpr (LOWER) {
+      uint32_t top = 0;
+      static_for<H>([&](auto ic) {
+        constexpr uint32_t M = slot_bit_mask<BW, J0, NJ, BW - 1, H0>(decltype(ic)::value);
+        if constexpr (M != 0) top += uint32_t(__builtin_popcount(hw(ic) & M));
+      });
+      cnt = uint32_t(NJ) - top;
+    } else {
+      cnt = NJ;
+    }
+    if constexpr (W == 64 && BW >= 32) {
+      uint64_t acc = 0;
+      emit_slot_words<UT, BW, J0, NJ>(w, [&](int, UT off) {
+        if constexpr (LOWER) acc += (off >> (BW - 1)) ? uint64_t(0) : uint64_t(off);
+        else acc += uint64_t(off);
+      });
+      offsum = acc;
+    } else if constexpr (LOWER && BW == 1) {
+      offsum = 0;  // a passing 1-bit offset is 0
+    } else if constexpr ((LOWER && BW <= 4) || (!LOWER && BW <= 2)) {
+      uint32_t acc = 0;
+      constexpr int NB = LOWER ? BW - 1 : BW;
+      static_for<NB>([&](auto bc) {
+        constexpr int b = decltype(bc)::value;
+        uint32_t c = 0;
+        static_for<H>([&](auto ic) {
+          constexpr int i = decltype(ic)::value;
+          constexpr uint32_t P = slot_bit_mask<BW, J0, NJ, b, H0>(i);
+          if constexpr (P != 0) {
+            uint32_t x = hw(ic) & P;
+            if constexpr (LOWER) {
+              const uint32_t t =
+                  __builtin_amdgcn_alignbit(hw(std::integral_constant<int, i + 1>{}), hw(ic),
+                                            uint32_t(BW - 1 - b));
+              x &= ~t;
+            }
+            c += uint32_t(__builtin_popcount(x));
+          }
+        });
+        acc += c << b;
+      });
+      offsum = acc;
+    } else {
+      constexpr bool FITS = LOWER ? (BW - 1 + ilog2(NJ) <= 32) : (BW + ilog2(NJ) <= 32);
+      using Acc = std::conditional_t<FITS, uint32_t, uint64_t>;
+      Acc acc = 0;
+      static_for<NJ>([&](auto jc) {
+        constexpr int q = (J0 + int(decltype(jc)::value)) * BW - 32 * H0;
+        constexpr int i = q / 32;
+        constexpr int s = q % 32;
+        uint32_t x;
+        if constexpr (s + BW <= 32) {
+          x = hw(std::integral_constant<int, i>{}) >> s;
+        } else {
+          x = __builtin_amdgcn_alignbit(hw(std::integral_constant<int, i + 1>{}),
+                                        hw(std::integral_constant<int, i>{}), uint32_t(s));
+        }
+        if constexpr (LOWER) {
+          const int32_t v = BW == 32 ? int32_t(x) : int32_t(x << (32 - BW)) >> (32 - BW);
+          acc += Acc(uint32_t(v > 0 ? v : 0));
+        } else {
+          acc += Acc(BW == 32 ? x : (x & ((1u << BW) - 1)));
+        }
+      });
+      offsum = acc;
+    }
+  }
+}
+
+// Filter + sum of slots [J0, J0 + NJ) of one lane (words in registers):
+// adds the count and sum (mod 2^64) of the passing values. For the offset
+// predicates the sum of the passing values is the sum of their offsets +
+// count * base, so the base is applied once: exact for every 64-bit vector
+// (sums are mod 2^64 anyway) and for a 32-bit vector whose base + 2^BW - 1
+// does not wrap; for one that does, a passing value base + off that wraps is
+// 2^32 less than the term counted, so the count of passing offsets >=
+// 2^32 - base (one compare per value) corrects the sum. The range predicate
+// compares offsets with the range shifted by the base while base + off
+// cannot wrap, and the wrapped value itself otherwise.
+template <typename UT, int BW, int J0, int NJ, int PRED>
+__device__ __forceinline__ void span_filter_sum(const UT *w, UT base, UT lo, UT hi,
+                                                uint64_t &count, uint64_t &sum) {
+  constexpr int W = lanes<UT>::W;
+  constexpr UT MAXOFF = BW == W ? UT(~UT(0)) : UT((UT(1) << BW) - 1);
+  const bool no_wrap = base <= UT(UT(~UT(0)) - MAXOFF);
+  if constexpr (PRED == FLSGPU_PRED_RANGE) {
+    using Acc = std::conditional_t<(W == 32 && BW + ilog2(NJ) <= 32), uint32_t, uint64_t>;
+    if (no_wrap) {
+      // lo <= base + off < hi  <=>  off - olo < span (unsigned)
+      const UT from = lo > base ? lo : base;
+      const UT olo = UT(from - base);
+      const UT span = hi > from ? UT(hi - from) : UT(0);
+      Acc acc = 0;
+      uint32_t cnt = 0;
+      emit_slot_words<UT, BW, J0, NJ>(w, [&](int, UT off) {
+        const bool pass = UT(off - olo) < span;
+        cnt += pass;
+        acc += pass ? Acc(off) : Acc(0);
+      });
+      count += cnt;
+      sum += uint64_t(acc) + uint64_t(cnt) * uint64_t(base);
+    } else {
+      emit_slot_words<UT, BW, J0, NJ>(w, [&](int, UT off) {
+        const UT v = UT(off + base);
+        const bool pass = (v >= lo) & (v < hi);
+        count += pass;
+        sum += pass ? uint64_t(v) : 0;
+      });
+    }
+  } else {
+    uint32_t cnt;
+    uint64_t offsum;
+    span_offset_stats<UT, BW, J0, NJ, PRED>(w, cnt, offsum);
+    count += cnt;
+    sum += offsum + uint64_t(cnt) * uint64_t(base);
+    if constexpr (W == 32 && BW > 0) {  // 64-bit sums are mod 2^64 anyway
+      if (!no_wrap) {
+        const UT first_wrap = UT(UT(0) - base);
+        uint32_t n_wrap = 0;
+        emit_slot_words<UT, BW, J0, NJ>(w, [&](int, UT off) {
+          bool pass = off >= first_wrap;
+          if constexpr (PRED == FLSGPU_PRED_OFFSET_LOWER_HALF) pass &= (off >> (BW - 1)) == 0;
+          n_wrap += pass;
+        });
+        sum -= uint64_t(n_wrap) << 32;
+      }
+    }
+  }
+}
+
+template <typename UT, int BW>
+__device__ __forceinline__ bool sum_no_wrap(UT base) {
+  constexpr int W = lanes<UT>::W;
+  constexpr UT MAXOFF = BW == W ? UT(~UT(0)) : UT((UT(1) << BW) - 1);
+  return W == 64 || base <= UT(UT(~UT(0)) - MAXOFF);
+}
 
 template <typename UT>
 struct ComputePolicy {
@@ -316,17 +435,249 @@ __global__ __launch_bounds__(BLOCK) void k_ffor_any(FforArgs<UT> a, UT magic, ui
   block_partial2<RedOp::Or>(found, 0, s.partials);
 }
 
-template <typename UT, int S, int PRED>
-__global__ __launch_bounds__(BLOCK) void k_ffor_filter_sum(FforArgs<UT> a, UT lo, UT hi,
-                                                          uint64_t n_waves, Scratch s) {
-  uint64_t count = 0, sum = 0;
-  for_each_unit_prefetched<UT, S>(a, n_waves, [&](const UnitMap<UT, S> &u, int bw, uint64_t off,
-                                                  UT base) {
-    FilterSumPolicy<UT, PRED> pol{a.packed + off + u.lane, base, lo, hi, 0, 0};
-    dispatch_bw<UT, S>(pol, bw, u.part);
-    count += pol.count;
-    sum += pol.sum;
+// Fused filter + sum. Every wave takes ranges of `units` consecutive units
+// (units * VPW vectors, at most 64): lane i loads vector i's bit width,
+// offset and base with one coalesced load each, and each unit's threads take
+// theirs by shuffle. When all of a range's vectors have one width, its units
+// go in batches of as many as fit about 32 registers of packed words, the
+// next batch's loads issued before the current batch is reduced (two
+// buffers), so a wave always has a batch of loads in flight; at width 1 a
+// whole range of 64 vectors is one batch of one word per thread. A lane's
+// words beyond that budget (64-bit types from width 17) are read in 2 or 4
+// chunks of slots, one unit at a time. Ranges of mixed widths go unit by
+// unit. The host picks `units` so that the ranges fill the resident waves
+// about once, and caps the grid at what fits on the device at once.
+template <typename UT>
+constexpr int SCAN_RANGE_UNITS = WAVE / lanes<UT>::VPW;  // 64 vectors
+
+constexpr int SCAN_DEAD = 255;
+
+__host__ __device__ constexpr int pow2_floor(int x) { return x < 2 ? 1 : 2 * pow2_floor(x / 2); }
+
+template <typename UT, int BW>
+struct ScanShape {
+  static constexpr int W = lanes<UT>::W;
+  static constexpr int LANE_DW = BW * W / 32;  // 32-bit words per lane
+  static constexpr int C = LANE_DW <= 32 ? 1 : (LANE_DW <= 64 ? 2 : 4);  // chunks per lane
+  static constexpr int NJ = W / C;                                       // slots per chunk
+  static constexpr int MAXKB = 8;  // each unit's address is held until its loads issue
+  static constexpr int FIT = LANE_DW == 0 ? MAXKB : 32 / LANE_DW;
+  static constexpr int KB = C > 1 ? 1 : pow2_floor(FIT < MAXKB ? FIT : MAXKB);  // units per batch
+};
+
+// A range's metadata, lane i holding vector i's.
+template <typename UT>
+struct RangeMeta {
+  uint64_t off;
+  UT base;
+  uint32_t nv;  // vectors in the range
+  int vsub;     // this thread's vector within a unit
+};
+
+// Whether this thread's vector of unit u of the range exists, and its offset
+// (safe_off, a vector of the same width, when it does not).
+template <typename UT>
+__device__ __forceinline__ bool range_unit_off(const RangeMeta<UT> &r, uint32_t u,
+                                               uint64_t safe_off, uint64_t &off) {
+  const uint32_t src = u * uint32_t(lanes<UT>::VPW) + uint32_t(r.vsub);
+  const uint64_t o = __shfl(r.off, int(src % WAVE), WAVE);
+  const bool live = src < r.nv;
+  off = live ? o : safe_off;
+  return live;
+}
+
+// The base of this thread's vector of unit u of the range.
+template <typename UT>
+__device__ __forceinline__ UT range_unit_base(const RangeMeta<UT> &r, uint32_t u) {
+  const uint32_t src = u * uint32_t(lanes<UT>::VPW) + uint32_t(r.vsub);
+  return __shfl(r.base, int(src % WAVE), WAVE);
+}
+
+// Filter + sum of one unit of width BW at `off` (adds nothing unless live),
+// its lane read in chunks of slots.
+template <typename UT, int PRED, int BW>
+__device__ __forceinline__ void unit_filter_sum(const FforArgs<UT> &a, uint64_t off, UT base,
+                                                bool live, int lane, UT lo, UT hi,
+                                                uint64_t &count, uint64_t &sum) {
+  using Sh = ScanShape<UT, BW>;
+  static_for<Sh::C>([&](auto cc) {
+    constexpr int J0 = int(decltype(cc)::value) * Sh::NJ;
+    UT w[SlotSpan<UT, BW, J0, Sh::NJ>::NR1];
+    load_slot_words<UT, BW, J0, Sh::NJ>(a.packed + off + lane, w);
+    uint64_t c = 0, t = 0;
+    span_filter_sum<UT, BW, J0, Sh::NJ, PRED>(w, base, lo, hi, c, t);
+    count += live ? c : 0;
+    sum += live ? t : 0;
   });
+}
+
+// Filter + sum of the n_units units of a range whose vectors all have width
+// BW (lane 0 holds the range's first vector).
+template <typename UT, int PRED, int BW>
+__device__ __forceinline__ void range_uniform(const FforArgs<UT> &a, const RangeMeta<UT> &r,
+                                              uint32_t n_units, int lane, UT lo, UT hi,
+                                              uint64_t &count, uint64_t &sum) {
+  constexpr int W = lanes<UT>::W;
+  using Sh = ScanShape<UT, BW>;
+  constexpr int KB = Sh::KB;
+  const uint64_t safe_off = uniform_u64(r.off);
+  if constexpr (Sh::C == 1) {
+    using Words = UT[KB][SlotSpan<UT, BW, 0, W>::NR1];
+    // Loads are unconditional (units past the range read the vector at
+    // safe_off and add nothing): a load under the branch that also guards a
+    // batch's arithmetic would be merged into it, serialising the batches.
+    // A unit's base is fetched when the batch is reduced, so a buffer holds
+    // only words.
+    auto load = [&](Words &w, uint32_t u0) {
+      static_for<KB>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        uint64_t off;
+        range_unit_off<UT>(r, u0 + k, safe_off, off);
+        load_slot_words<UT, BW, 0, W>(a.packed + off + lane, w[k]);
+      });
+    };
+    auto reduce = [&](const Words &w, uint32_t u0) {
+      static_for<KB>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const uint32_t src = (u0 + k) * uint32_t(lanes<UT>::VPW) + uint32_t(r.vsub);
+        uint64_t c = 0, t = 0;
+        span_filter_sum<UT, BW, 0, W, PRED>(w[k], range_unit_base<UT>(r, u0 + k), lo, hi, c, t);
+        count += src < r.nv ? c : 0;
+        sum += src < r.nv ? t : 0;
+      });
+    };
+    // Every load is issued unconditionally, past the range too: after a
+    // branch that may or may not issue loads the compiler waits for all of
+    // them (the load counter drains in issue order), so a guarded prefetch
+    // would make the wait for the current batch also wait for the next one.
+    const uint32_t nb = (n_units + KB - 1) / KB;
+    Words w0, w1;
+    load(w0, 0);
+#pragma unroll 1
+    for (uint32_t b = 0; b < nb; b += 2) {
+      load(w1, (b + 1) * KB);
+      reduce(w0, b * KB);
+      load(w0, (b + 2) * KB);
+      reduce(w1, (b + 1) * KB);
+    }
+  } else {
+#pragma unroll 1
+    for (uint32_t u = 0; u < n_units; ++u) {
+      uint64_t off;
+      const bool live = range_unit_off<UT>(r, u, safe_off, off);
+      unit_filter_sum<UT, PRED, BW>(a, off, range_unit_base<UT>(r, u), live, lane, lo, hi, count,
+                                    sum);
+    }
+  }
+}
+
+#define FLSGPU_FS_RANGE_CASE(B)                                                  \
+  case B:                                                                       \
+    if constexpr ((B) <= lanes<UT>::W)                                           \
+      range_uniform<UT, PRED, (B)>(a, r, n_units, lane, lo, hi, count, sum);     \
+    break;
+#define FLSGPU_FS_UNIT_CASE(B)                                                   \
+  case B:                                                                       \
+    if constexpr ((B) <= lanes<UT>::W)                                           \
+      unit_filter_sum<UT, PRED, (B)>(a, off, base, live, lane, lo, hi, count, sum); \
+    break;
+
+template <typename UT, int PRED>
+__device__ __forceinline__ void range_dispatch(const FforArgs<UT> &a, const RangeMeta<UT> &r,
+                                               uint32_t n_units, int bw, int lane, UT lo, UT hi,
+                                               uint64_t &count, uint64_t &sum) {
+  if constexpr (lanes<UT>::W == 32) {
+    switch (bw) {
+      FLSGPU_CASES_0_32(FLSGPU_FS_RANGE_CASE)
+      default: break;
+    }
+  } else {
+    switch (bw) {
+      FLSGPU_CASES_0_32(FLSGPU_FS_RANGE_CASE)
+      FLSGPU_CASES_33_64(FLSGPU_FS_RANGE_CASE)
+      default: break;
+    }
+  }
+}
+
+template <typename UT, int PRED>
+__device__ __forceinline__ void unit_dispatch(const FforArgs<UT> &a, uint64_t off, UT base,
+                                              bool live, int bw, int lane, UT lo, UT hi,
+                                              uint64_t &count, uint64_t &sum) {
+  if constexpr (lanes<UT>::W == 32) {
+    switch (bw) {
+      FLSGPU_CASES_0_32(FLSGPU_FS_UNIT_CASE)
+      default: break;
+    }
+  } else {
+    switch (bw) {
+      FLSGPU_CASES_0_32(FLSGPU_FS_UNIT_CASE)
+      FLSGPU_CASES_33_64(FLSGPU_FS_UNIT_CASE)
+      default: break;
+    }
+  }
+}
+#undef FLSGPU_FS_RANGE_CASE
+#undef FLSGPU_FS_UNIT_CASE
+
+// Waves per SIMD the register allocator is held to (the offset-statistics
+// kernels fit without spilling vector registers; the range kernel is left
+// unconstrained).
+template <typename UT, int PRED>
+constexpr int FS_WAVES = PRED == FLSGPU_PRED_RANGE ? 1 : 3;
+
+template <typename UT, int S, int PRED>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(FS_WAVES<UT, PRED>, 8)))
+void k_ffor_filter_sum(FforArgs<UT> a, UT lo, UT hi, uint32_t units, Scratch s) {
+  static_assert(S == 1, "the fused filter + sum reads whole lanes");
+  using L = lanes<UT>;
+  const int t = int(threadIdx.x % WAVE);
+  const int lane = t % L::N;
+  const uint64_t span = uint64_t(units) * L::VPW;  // vectors per range
+  const uint64_t n_ranges = (a.n_vecs + span - 1) / span;
+  const uint64_t stride = uint64_t(gridDim.x) * (BLOCK / WAVE);
+  uint64_t count = 0, sum = 0;
+  for (uint64_t rr = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE; rr < n_ranges;
+       rr += stride) {
+    const uint64_t v0 = rr * span;
+    RangeMeta<UT> r;
+    r.nv = uint32_t(a.n_vecs - v0 < span ? a.n_vecs - v0 : span);
+    r.vsub = t / L::N;
+    int mbw = SCAN_DEAD;
+    r.off = 0;
+    r.base = UT(0);
+    if (uint32_t(t) < r.nv) {
+      mbw = a.bit_widths[v0 + t];
+      r.off = a.vector_offsets[v0 + t];
+      r.base = a.bases ? a.bases[v0 + t] : UT(0);
+    }
+    const uint32_t n_units = (r.nv + L::VPW - 1) / L::VPW;
+    // Lane 0 holds the range's first vector.
+    const int b0 = __builtin_amdgcn_readfirstlane(mbw);
+    if (__builtin_amdgcn_ballot_w64(mbw != b0 && mbw != SCAN_DEAD) == 0) {
+      range_dispatch<UT, PRED>(a, r, n_units, b0, lane, lo, hi, count, sum);
+    } else {
+#pragma unroll 1
+      for (uint32_t u = 0; u < n_units; ++u) {
+        const uint32_t src = u * uint32_t(L::VPW) + uint32_t(r.vsub);
+        const int bw = __shfl(mbw, int(src % WAVE), WAVE);  // SCAN_DEAD past the range
+        const uint64_t first = uniform_u64(__shfl(r.off, int(u * L::VPW), WAVE));
+        uint64_t off;
+        const bool live = range_unit_off<UT>(r, u, first, off);
+        const UT base = range_unit_base<UT>(r, u);
+        // The unit's first vector (lane 0's) exists.
+        const int b1 = __builtin_amdgcn_readfirstlane(bw);
+        if (__builtin_amdgcn_ballot_w64(live && bw != b1) == 0) {
+          unit_dispatch<UT, PRED>(a, off, base, live, b1, lane, lo, hi, count, sum);
+        } else if (live && bw <= L::W) {
+          FilterSumPolicy<UT, PRED> pol{a.packed + off + lane, base, lo, hi, 0, 0};
+          pol.run_generic(bw, 0, L::W);
+          count += pol.count;
+          sum += pol.sum;
+        }
+      }
+    }
+  }
   block_partial2<RedOp::Add>(count, sum, s.partials);
 }
 
@@ -427,12 +778,26 @@ template <typename UT, int S, int PRED>
 int FlsFilterSum<UT, S, PRED>::launch_partials(const FforArgs<UT> &a, UT lo, UT hi,
                                                uint64_t *d_result, uint64_t *partials,
                                                unsigned slots, hipStream_t stream) {
-  const uint64_t n_waves = n_waves_for<UT, S>(a.n_vecs);
-  const unsigned grid = std::min({reduce_grid(n_waves), SCAN_GRID, slots});
+  static const unsigned resident = [] {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ffor_filter_sum<UT, S, PRED>, BLOCK,
+                                                     0) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    return unsigned(per_cu) * unsigned(num_cus());
+  }();
+  // Units per range: the ranges fill the resident waves about once.
+  const uint64_t n_units = n_waves_for<UT, 1>(a.n_vecs);
+  const uint64_t waves = uint64_t(resident) * (BLOCK / WAVE);
+  const uint32_t units = uint32_t(
+      std::clamp<uint64_t>((n_units + waves - 1) / waves, 1, SCAN_RANGE_UNITS<UT>));
+  const uint64_t n_ranges = (n_units + units - 1) / units;
+  const uint64_t blocks = (n_ranges + (BLOCK / WAVE) - 1) / (BLOCK / WAVE);
+  const unsigned grid = unsigned(
+      std::min<uint64_t>({std::max<uint64_t>(blocks, 1), resident, SCAN_GRID, slots}));
   if (grid == 0) return set_error(FLSGPU_EINVAL, "no room for block partials");
   const Scratch s{partials};
   hipLaunchKernelGGL((k_ffor_filter_sum<UT, S, PRED>), dim3(grid), dim3(BLOCK), 0, stream, a, lo,
-                     hi, n_waves, s);
+                     hi, units, s);
   int rc = check_hip(hipGetLastError(), "k_ffor_filter_sum launch");
   if (rc) return rc;
   return reduce_finish_pair(s, grid, d_result, stream);

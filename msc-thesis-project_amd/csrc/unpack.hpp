@@ -36,32 +36,38 @@ struct NoHook {
   __device__ __forceinline__ void operator()() const {}
 };
 
-// Calls emit(j, value) for slots J0..J0+NJ-1 of one lane. `in` points at the
-// lane's word 0 of the vector (packed + vector_offset + lane). `after_loads`
-// runs once every packed word load has been issued and before any is used:
-// loads it issues are in flight together with the packed words, and waiting
-// for the packed words does not wait for them first (the load counter drains
-// in issue order).
-template <typename UT, int BW, int J0, int NJ, class Emit, class Hook = NoHook>
-__device__ __forceinline__ void unpack_slots(const UT *__restrict in, Emit &&emit,
-                                             Hook &&after_loads = Hook{}) {
+// The words of one lane that slots [J0, J0 + NJ) at bit width BW touch:
+// lane words R0 .. R0 + NR - 1 (none at width 0).
+template <typename UT, int BW, int J0, int NJ>
+struct SlotSpan {
+  static constexpr int W = lanes<UT>::W;
+  static constexpr int R0 = BW == 0 ? 0 : (J0 * BW) / W;
+  static constexpr int NR = BW == 0 ? 0 : ((J0 + NJ) * BW - 1) / W - R0 + 1;
+  static constexpr int NR1 = NR > 0 ? NR : 1;  // array extent
+};
+
+// Loads the span's words of one lane; `in` points at the lane's word 0 of the
+// vector (packed + vector_offset + lane).
+template <typename UT, int BW, int J0, int NJ>
+__device__ __forceinline__ void load_slot_words(const UT *__restrict in, UT *w) {
+  using Sp = SlotSpan<UT, BW, J0, NJ>;
+  static_for<Sp::NR>([&](auto rc) {
+    constexpr int r = decltype(rc)::value;
+    w[r] = in[(Sp::R0 + r) * lanes<UT>::N];
+  });
+}
+
+// Calls emit(j, value) for slots J0..J0+NJ-1 from the span's words, already
+// in registers (w[0] = lane word R0).
+template <typename UT, int BW, int J0, int NJ, class Emit>
+__device__ __forceinline__ void emit_slot_words(const UT *w, Emit &&emit) {
   constexpr int W = lanes<UT>::W;
-  constexpr int NL = lanes<UT>::N;
   static_assert(BW >= 0 && BW <= W, "bit width out of range");
   if constexpr (BW == 0) {
-    after_loads();
     static_for<NJ>([&](auto jc) { emit(J0 + int(decltype(jc)::value), UT(0)); });
   } else {
-    constexpr int R0 = (J0 * BW) / W;
-    constexpr int R1 = ((J0 + NJ) * BW - 1) / W;
-    constexpr int NR = R1 - R0 + 1;
+    constexpr int R0 = SlotSpan<UT, BW, J0, NJ>::R0;
     constexpr UT MASK = BW == W ? UT(~UT(0)) : UT((UT(1) << BW) - 1);
-    UT w[NR];
-    static_for<NR>([&](auto rc) {
-      constexpr int r = decltype(rc)::value;
-      w[r] = in[(R0 + r) * NL];
-    });
-    after_loads();
     static_for<NJ>([&](auto jc) {
       constexpr int j = J0 + int(decltype(jc)::value);
       constexpr int p = j * BW;
@@ -77,6 +83,21 @@ __device__ __forceinline__ void unpack_slots(const UT *__restrict in, Emit &&emi
       emit(j, v);
     });
   }
+}
+
+// Calls emit(j, value) for slots J0..J0+NJ-1 of one lane. `in` points at the
+// lane's word 0 of the vector (packed + vector_offset + lane). `after_loads`
+// runs once every packed word load has been issued and before any is used:
+// loads it issues are in flight together with the packed words, and waiting
+// for the packed words does not wait for them first (the load counter drains
+// in issue order).
+template <typename UT, int BW, int J0, int NJ, class Emit, class Hook = NoHook>
+__device__ __forceinline__ void unpack_slots(const UT *__restrict in, Emit &&emit,
+                                             Hook &&after_loads = Hook{}) {
+  UT w[SlotSpan<UT, BW, J0, NJ>::NR1];
+  load_slot_words<UT, BW, J0, NJ>(in, w);
+  after_loads();
+  emit_slot_words<UT, BW, J0, NJ>(w, emit);
 }
 
 // Dispatch on a wave-uniform bit width and slot part. Policy provides

@@ -204,6 +204,39 @@ def test_filter_sum(width):
                 assert int(r[1]) == sum(int(x) for x in dec[sel]) & m64, (tag, lo, hi)
 
 
+def _sum_mod64(x):
+    """Sum mod 2^64 of unsigned values (numpy's uint64 sum wraps)."""
+    return int(np.asarray(x, dtype=np.uint64).sum(dtype=np.uint64))
+
+
+@pytest.mark.parametrize("width", [32, 64])
+def test_filter_sum_every_width(width):
+    """Every bit width 0..W through the fused kernel's batched path: columns of
+    64 vectors (every wave's group of units full) and of 37 (a ragged last
+    group), bases small (offset + base never wraps) and full-range (wrapping
+    32-bit values take the per-value correction), each predicate against
+    closed forms on the oracle's decode."""
+    rng = np.random.default_rng(21)
+    for bw in range(width + 1):
+        for n_vecs, small_bases in ((64, True), (37, False), (64, False)):
+            packed, bws, offs, bases = random_column(rng, n_vecs, width, [bw] * n_vecs)
+            if small_bases:
+                bases = rng.integers(0, 1 << 20, n_vecs).astype(UT[width])
+            col = gpu_column(packed, bws, offs, bases, width)
+            dec = oracle_decode(packed, bws, offs, bases, width)
+            off = (dec.reshape(n_vecs, 1024) - bases[:, None]).reshape(-1)
+            tag = f"u{width} bw={bw} n_vecs={n_vecs} small_bases={small_bases}"
+            sel = (off < (1 << (bw - 1))) if bw > 0 else np.zeros(len(dec), bool)
+            r = D.query_filter_sum(col, A.PRED_OFFSET_LOWER_HALF).cpu().numpy()
+            assert (int(r[0]), int(r[1])) == (int(sel.sum()), _sum_mod64(dec[sel])), tag
+            r = D.query_filter_sum(col, A.PRED_ALL).cpu().numpy()
+            assert (int(r[0]), int(r[1])) == (len(dec), _sum_mod64(dec)), tag
+            lo, hi = sorted(int(x) for x in rng.choice(dec, 2))
+            sel = (dec >= UT[width](lo)) & (dec < UT[width](hi))
+            r = D.query_filter_sum(col, A.PRED_RANGE, lo, hi).cpu().numpy()
+            assert (int(r[0]), int(r[1])) == (int(sel.sum()), _sum_mod64(dec[sel])), (tag, lo, hi)
+
+
 @pytest.mark.parametrize("width", [32, 64])
 def test_compute_is_zero(width):
     # kernels.cuh:56-89: every value is &= 0 each repetition, so "== 1" never holds.
