@@ -95,6 +95,15 @@ inline T __shfl_xor(T v, int mask, int width = 64) {
   return emu_exchange(v, threadIdx.x ^ unsigned(mask));
 }
 template <typename T>
+inline T __shfl(T v, int src, int width = 64) {
+  const unsigned t = threadIdx.x, s = t - t % unsigned(width) + unsigned(src) % unsigned(width);
+  if (s / unsigned(g_emu.group) != t / unsigned(g_emu.group)) {
+    std::fprintf(stderr, "emulation: shfl source %u leaves the %d-thread group\n", s, g_emu.group);
+    std::abort();
+  }
+  return emu_exchange(v, s);
+}
+template <typename T>
 inline T __shfl_up(T v, unsigned delta, int width = 64) {
   if (width != g_emu.group) {
     std::fprintf(stderr, "emulation: shfl_up width %d, group %d\n", width, g_emu.group);
