@@ -200,6 +200,47 @@ def test_query_any(width):
 
 
 @pytest.mark.parametrize("width", [32, 64])
+def test_query_any_against_oracle(width):
+    """query_any at every kind of needle, against the oracle's decode: bases
+    small, negative and close to the signed maximum (where (base + v) * FACT
+    can wrap, so the kernel decodes every value instead of searching the
+    interval of matching offsets), widths 0 to W; needles taken from decoded
+    values (the interval's ends included), from exceptions and from between
+    two decoded values, plus NaN, infinities and signed zeros."""
+    rng = np.random.default_rng(93)
+    smax = (1 << (width - 1)) - 1
+    for bw in (0, 1, 2, 5, 13, 21, width - 2, width - 1, width):
+        for kind in ("small", "negative", "near_max"):
+            n_vecs = 4
+            col = reference_style_column(rng, width, n_vecs, [bw] * n_vecs, 6)
+            if kind == "negative":
+                col.bases[:] = np.array([-1000, -3, -(1 << 20), -7], np.int64).astype(UT[width])
+            elif kind == "near_max":
+                col.bases[:] = np.array([smax - 5, smax // 3, smax // 10 ** 3, 7], dtype=UT[width])
+            raw = oracle_decode(col, patch=False)
+            exp_vals = oracle_decode(col)
+            needles = [raw[int(i)] for i in rng.choice(len(raw), 6, replace=False)]
+            for v in range(n_vecs):  # each vector's smallest and largest decoded value
+                blk = raw[v * 1024:(v + 1) * 1024]
+                fin = blk[np.isfinite(blk)]
+                if len(fin):
+                    needles += [fin.min(), fin.max()]
+            if len(col.exceptions):
+                needles.append(col.exceptions[0])
+            srt = np.unique(raw[np.isfinite(raw)])
+            if len(srt) > 1:
+                needles.append(FT[width]((float(srt[0]) + float(srt[1])) / 2))
+            needles += [FT[width](np.nan), FT[width](np.inf), FT[width](-np.inf), FT[width](0.0),
+                        FT[width](-0.0)]
+            for layout_col in (col, AL.galp_convert(col)):
+                d = AL.DeviceAlpColumn.from_host(layout_col)
+                for nd in needles:
+                    want = int(bool((exp_vals == nd).any()))
+                    got = AL.query_any(d, float(nd)).item()
+                    assert got == want, (width, bw, kind, float(nd))
+
+
+@pytest.mark.parametrize("width", [32, 64])
 def test_multi_column(width):
     rng = np.random.default_rng(92)
     n_vecs = 4
