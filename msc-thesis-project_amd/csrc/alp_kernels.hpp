@@ -663,76 +663,501 @@ __device__ __forceinline__ uint64_t alp_lane_match(const AlpArgs<T> &a, uint64_t
   return m;
 }
 
+// Smallest v in [0, n] with p(v), for p false then true over [0, n] and p(n)
+// true: a gallop from the estimate e (steps 1, 2, 4, ... away from it), then
+// a binary search of the bracket. At most about 2 log2(n) evaluations.
+template <class P>
+__device__ __forceinline__ int64_t first_true(int64_t e, int64_t n, P &&p) {
+  int64_t f, t;  // p(f) false (f = -1: none), p(t) true
+  if (p(e)) {
+    t = e;
+    f = -1;
+    for (int64_t d = 1; t > 0; d *= 2) {
+      const int64_t x = t - d < 0 ? 0 : t - d;
+      if (!p(x)) {
+        f = x;
+        break;
+      }
+      t = x;
+    }
+  } else {
+    f = e;
+    t = n;
+    for (int64_t d = 1; f + d < n; d *= 2) {
+      if (p(f + d)) {
+        t = f + d;
+        break;
+      }
+      f += d;
+    }
+  }
+  while (t - f > 1) {
+    const int64_t m = f + (t - f) / 2;
+    if (p(m)) t = m;
+    else f = m;
+  }
+  return t;
+}
+
+// Largest v in [0, n] with p(v), for p true then false over [0, n] and p(0)
+// true (the mirror of first_true).
+template <class P>
+__device__ __forceinline__ int64_t last_true(int64_t e, int64_t n, P &&p) {
+  int64_t t, f;  // p(t) true, p(f) false (f = n + 1: none)
+  if (p(e)) {
+    t = e;
+    f = n + 1;
+    for (int64_t d = 1; t < n; d *= 2) {
+      const int64_t x = t + d > n ? n : t + d;
+      if (!p(x)) {
+        f = x;
+        break;
+      }
+      t = x;
+    }
+  } else {
+    f = e;
+    t = 0;
+    for (int64_t d = 1; f - d > 0; d *= 2) {
+      if (p(f - d)) {
+        t = f - d;
+        break;
+      }
+      f -= d;
+    }
+  }
+  while (f - t > 1) {
+    const int64_t m = t + (f - t) / 2;
+    if (p(m)) t = m;
+    else f = m;
+  }
+  return t;
+}
+
+// How the query matches the slots of one vector (before patching).
+enum AnyHow : int {
+  ANY_NONE = 0,      // no slot decodes to the needle
+  ANY_INTERVAL = 1,  // exactly the offsets v with v - lo <= span (unsigned)
+  ANY_DECODE = 2,    // decode and compare every slot
+  ANY_ALL = 3,       // every slot (a width above W decodes to zeros; the needle is zero)
+};
+
+// The FOR offsets v in [0, 2^bw) of a vector whose value decodes exactly to
+// `target`. When (base + v) * FACT cannot leave the signed range over the
+// vector's offsets, the decode's integer is the exact product, increasing in
+// v, and the RNE conversion and the multiply by the positive fraction keep
+// the order, so those offsets are one interval. Its ends are found with the
+// decode formula itself, so they agree with it bit for bit: an estimate from
+// the target, then a gallop and a binary search. Otherwise ANY_DECODE.
+template <typename T>
+__device__ __forceinline__ int alp_target_interval(const AlpConsts<T> &c, T target, int bw,
+                                                   typename uint_of<T>::type &lo,
+                                                   typename uint_of<T>::type &span) {
+  using UT = typename uint_of<T>::type;
+  using ST = typename int_of<T>::type;
+  constexpr int W = lanes<UT>::W;
+  if (bw > W) return T(0) == target ? ANY_ALL : ANY_NONE;
+  if (bw >= W - 1) return ANY_DECODE;
+  const UT v1 = UT((UT(1) << bw) - 1);  // the largest offset
+  constexpr ST ST_MAX = ST(UT(~UT(0)) >> 1);
+  const ST y0 = ST(c.base);
+  const ST f = ST(c.fact);
+  ST p0, p1;
+  if (f <= 0 || y0 > ST(ST_MAX - ST(v1)) || __builtin_mul_overflow(y0, f, &p0) ||
+      __builtin_mul_overflow(ST(y0 + ST(v1)), f, &p1))
+    return ANY_DECODE;
+  if (!(target >= c(UT(0)) && target <= c(v1))) return ANY_NONE;  // NaN too
+  const double q = double(target) / double(c.frac) / double(f) - double(y0);
+  const int64_t e = q <= 0.0 ? 0 : (q >= double(v1) ? int64_t(v1) : int64_t(q));
+  const int64_t first = first_true(e, int64_t(v1), [&](int64_t v) { return c(UT(v)) >= target; });
+  const int64_t last = last_true(e, int64_t(v1), [&](int64_t v) { return c(UT(v)) <= target; });
+  if (first > last) return ANY_NONE;
+  lo = UT(first);
+  span = UT(last - first);
+  return ANY_INTERVAL;
+}
+
 // "Any value == magic" over a patched column. The patched column holds the
 // exception values at their positions and the decoded values elsewhere, so
-// the answer is: some exception value equals magic, or some decoded value
-// at a position that is not an exception does. The first part reads the
-// exception values only (the vector's threads together, coalesced); the
-// positions are read only by a lane with a decoded match, to drop matches at
-// exception positions, which is rare. (Building the standard layout's LDS
-// index for every vector, as the full patch does, made the query slower than
-// the decode: 2^26 doubles, 91 us.) A position listed twice counts each of
-// its exception values.
-// Threads per lane of the query: two for doubles (the 64-slot decode and
-// compare otherwise needs ~184 VGPRs), one for floats.
+// the answer is: some exception value equals magic, or some decoded value at
+// a position that is not an exception does.
+//
+// Work mapping (as for the fused filter + sum of the FFOR kernels): every
+// wave takes ranges of up to 64 consecutive vectors. Lane i loads vector i's
+// metadata (coalesced) and works out once which of its offsets decode to the
+// needle (alp_target_interval): for the common case of a vector whose
+// values cannot reach the needle there is nothing to compare, and otherwise
+// one interval test per slot, not a decode. The range's exception values,
+// one contiguous run when the vectors' lists follow each other, are scanned
+// by the whole wave. The units of a range (VPW vectors each, one thread per
+// lane) then load their packed words in batches of as many as fit about 32
+// registers, the next batch's loads issued before the current one is
+// tested, and take their vector's answer by shuffle. A lane with a decoded
+// match reads its vector's exception positions to drop matches at exception
+// slots, which is rare. A position listed twice counts each of its
+// exception values.
 template <typename T>
-constexpr int alp_any_split() { return sizeof(T) == 8 ? 2 : 1; }
-
-template <typename T, int LAYOUT, bool PATCH>
-__global__ __launch_bounds__(BLOCK) void k_alp_any(AlpArgs<T> a, T magic, uint64_t n_waves,
-                                                  Scratch s) {
+struct AnyLane {
   using UT = typename uint_of<T>::type;
-  using L = lanes<UT>;
-  constexpr int S = alp_any_split<T>();
-  constexpr int NJ = L::W / S;
-  constexpr uint64_t FULL = NJ == 64 ? ~uint64_t(0) : ((uint64_t(1) << NJ) - 1);
-  bool found = false;
-  const uint64_t stride = uint64_t(gridDim.x) * (BLOCK / WAVE);
-  for (uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE; wave < n_waves;
-       wave += stride) {
-    const UnitMap<UT, S> u = map_unit<UT, S>(wave);
-    if (u.vec >= a.ffor.n_vecs) continue;
-    const int j0 = u.part * NJ;
-    AlpMatchPolicy<T> pol{a.ffor.packed + a.ffor.vector_offsets[u.vec] + u.lane,
-                          alp_consts(a, u.vec), magic, 0};
-    if (!dispatch_bw<UT, S>(pol, a.ffor.bit_widths[u.vec], u.part))
-      pol.mask = T(0) == magic ? FULL : 0;
-    uint64_t m = pol.mask;  // bit (j - j0): slot j decodes to magic
-    if constexpr (PATCH) {
-      const uint64_t eo = a.exceptions_offsets[u.vec];
-      unsigned count, first, k0, step;
-      if constexpr (LAYOUT == ALP_GALP) {
-        const unsigned oc = a.offsets_counts[u.vec * L::N + u.lane];
-        count = oc >> 10;
-        first = oc & 0x3FFu;
-        k0 = 0;  // the lane's own list
-        step = 1;
-      } else {
-        count = a.counts[u.vec];
-        first = 0;
-        k0 = unsigned(u.lane);  // the vector's list, shared by its lanes
-        step = unsigned(L::N);
-      }
-      if (u.part == 0) {  // the exception values, ALP_PF loads in flight
-        for (unsigned k = k0; k < count; k += ALP_PF * step) {
-          T e[ALP_PF];
-#pragma unroll
-          for (int i = 0; i < ALP_PF; ++i)
-            e[i] = k + i * step < count ? a.exceptions[eo + first + k + i * step] : T(0);
-#pragma unroll
-          for (int i = 0; i < ALP_PF; ++i) found |= k + i * step < count && e[i] == magic;
+  uint64_t off, eo;
+  UT base, lo, span, fact;
+  T frac;
+  int how;
+  unsigned cnt;  // the vector's exception count
+};
+
+template <typename UT, int BW>
+struct AnyShape {
+  static constexpr int W = lanes<UT>::W;
+  static constexpr int LANE_DW = BW * W / 32;  // 32-bit words per lane
+  static constexpr int C = LANE_DW <= 32 ? 1 : (LANE_DW <= 64 ? 2 : 4);  // chunks per lane
+  static constexpr int NJ = W / C;                                       // slots per chunk
+  // units per batch: their words within about 32 registers and their slots
+  // within 128 (each slot may need a decode and compare)
+  static constexpr int MAXKB = 128 / W;
+  static constexpr int FIT = LANE_DW == 0 ? MAXKB : 32 / LANE_DW;
+  static constexpr int KB = C > 1 ? 1 : (FIT >= MAXKB ? MAXKB : (FIT >= 2 ? 2 : 1));
+};
+
+constexpr int ANY_DEAD_WIDTH = 255;  // a lane past the range
+
+// This thread's lane of the vector at `off`.
+template <typename T>
+__device__ __forceinline__ const typename uint_of<T>::type *generic_in(const AlpArgs<T> &a,
+                                                                      uint64_t off, int lane) {
+  return a.ffor.packed + off + lane;
+}
+
+// Lane of the range holding this thread's vector of unit u.
+template <typename UT>
+__device__ __forceinline__ uint32_t any_src(uint32_t u, int vsub) {
+  return u * uint32_t(lanes<UT>::VPW) + uint32_t(vsub);
+}
+
+// Match mask of slots [J0, J0 + NJ) (bit j - J0) from the lane's words, for
+// the vector whose answer is held by lane `sl` (shuffled in here; every
+// thread of the wave takes part).
+template <typename T, int BW, int J0, int NJ>
+__device__ __forceinline__ uint64_t any_mask(const typename uint_of<T>::type *w,
+                                             const AnyLane<T> &l, int sl, bool live, T magic) {
+  using UT = typename uint_of<T>::type;
+  const int how = live ? __shfl(l.how, sl, WAVE) : int(ANY_NONE);
+  const UT lo = __shfl(l.lo, sl, WAVE), span = __shfl(l.span, sl, WAVE);
+  AlpConsts<T> c{__shfl(l.base, sl, WAVE), 0, 0, __shfl(l.fact, sl, WAVE), __shfl(l.frac, sl, WAVE)};
+  uint64_t m = 0;
+  if (how == ANY_INTERVAL) {
+    emit_slot_words<UT, BW, J0, NJ>(w, [&](int j, UT v) { m |= uint64_t(UT(v - lo) <= span) << (j - J0); });
+  } else if (how == ANY_DECODE) {
+    emit_slot_words<UT, BW, J0, NJ>(w, [&](int j, UT v) { m |= uint64_t(c(v) == magic) << (j - J0); });
+  } else if (how == ANY_ALL) {
+    m = NJ == 64 ? ~uint64_t(0) : ((uint64_t(1) << NJ) - 1);
+  }
+  return m;
+}
+
+// Drops from m (slots [j0, j0 + nj) of the thread's lane) the slots that are
+// exceptions of its vector.
+template <typename T, int LAYOUT>
+__device__ __forceinline__ uint64_t any_drop_exceptions(const AlpArgs<T> &a, uint64_t vec,
+                                                        int lane, int j0, int nj, uint64_t m,
+                                                        uint64_t eo, unsigned cnt) {
+  constexpr int NL = lanes<typename uint_of<T>::type>::N;
+  uint64_t first = eo;
+  unsigned count = cnt;
+  if constexpr (LAYOUT == ALP_GALP) {  // the lane's own list
+    const unsigned oc = a.offsets_counts[vec * NL + lane];
+    first = eo + (oc & 0x3FFu);
+    count = oc >> 10;
+  }
+  for (unsigned k = 0; k < count; ++k) {
+    const unsigned pos = unsigned(a.positions[first + k]) & 1023u;
+    const int slot = int(pos / unsigned(NL));
+    if (int(pos % unsigned(NL)) == lane && slot >= j0 && slot < j0 + nj)
+      m &= ~(uint64_t(1) << (slot - j0));
+  }
+  return m;
+}
+
+// Tests one unit's (already loaded) chunk and folds the answer into found.
+template <typename T, int LAYOUT, bool PATCH, int BW, int J0, int NJ>
+__device__ __forceinline__ void any_test(const AlpArgs<T> &a, const typename uint_of<T>::type *w,
+                                         const AnyLane<T> &l, uint64_t v0, uint32_t nv, uint32_t u,
+                                         int vsub, int lane, T magic, bool &found) {
+  using UT = typename uint_of<T>::type;
+  const uint32_t src = any_src<UT>(u, vsub);
+  const int sl = int(src % WAVE);
+  const bool live = src < nv;
+  uint64_t m = any_mask<T, BW, J0, NJ>(w, l, sl, live, magic);
+  if constexpr (PATCH) {
+    if (__builtin_amdgcn_ballot_w64(m != 0)) {  // rare: every thread takes part in the shuffles
+      const uint64_t eo = __shfl(l.eo, sl, WAVE);
+      const unsigned cnt = __shfl(l.cnt, sl, WAVE);
+      if (m) m = any_drop_exceptions<T, LAYOUT>(a, v0 + src, lane, J0, NJ, m, eo, cnt);
+    }
+  }
+  found |= m != 0;
+}
+
+// The range's units when all of its vectors have width BW (lane 0 holds the
+// range's first vector).
+template <typename T, int LAYOUT, bool PATCH, int BW>
+__device__ __forceinline__ void any_range_uniform(const AlpArgs<T> &a, const AnyLane<T> &l,
+                                                  uint64_t v0, uint32_t nv, uint32_t n_units,
+                                                  int vsub, int lane, T magic, bool &found) {
+  using UT = typename uint_of<T>::type;
+  constexpr int W = lanes<UT>::W;
+  using Sh = AnyShape<UT, BW>;
+  constexpr int KB = Sh::KB;
+  const uint64_t safe_off = uniform_u64(l.off);
+  // A unit past the range reads the vector at safe_off (width BW) and tests
+  // nothing; loads are never under a branch (see the fused filter + sum).
+  auto unit_off = [&](uint32_t u) {
+    const uint32_t src = any_src<UT>(u, vsub);
+    const uint64_t o = __shfl(l.off, int(src % WAVE), WAVE);
+    return src < nv ? o : safe_off;
+  };
+  if constexpr (Sh::C == 1) {
+    using Words = UT[KB][SlotSpan<UT, BW, 0, W>::NR1];
+    auto load = [&](Words &w, uint32_t u0) {
+      static_for<KB>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        load_slot_words<UT, BW, 0, W>(generic_in(a, unit_off(u0 + k), lane), w[k]);
+      });
+    };
+    // The batch's masks first; the rare exception-slot check after them, so
+    // that its registers do not overlap the words'.
+    auto test = [&](const Words &w, uint32_t u0) {
+      uint64_t m[KB];
+      bool any = false;
+      static_for<KB>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const uint32_t src = any_src<UT>(u0 + k, vsub);
+        m[k] = any_mask<T, BW, 0, W>(w[k], l, int(src % WAVE), src < nv, magic);
+        any |= m[k] != 0;
+      });
+      if constexpr (PATCH) {
+        if (__builtin_amdgcn_ballot_w64(any)) {
+          static_for<KB>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            const uint32_t src = any_src<UT>(u0 + k, vsub);
+            const uint64_t eo = __shfl(l.eo, int(src % WAVE), WAVE);
+            const unsigned cnt = __shfl(l.cnt, int(src % WAVE), WAVE);
+            if (m[k]) m[k] = any_drop_exceptions<T, LAYOUT>(a, v0 + src, lane, 0, W, m[k], eo, cnt);
+          });
         }
       }
-      if (m) {
-        for (unsigned k = 0; k < count; ++k) {
-          const unsigned pos = unsigned(a.positions[eo + first + k]) & 1023u;
-          const int slot = int(pos / unsigned(L::N));
-          if (int(pos % unsigned(L::N)) == u.lane && slot >= j0 && slot < j0 + NJ)
-            m &= ~(uint64_t(1) << (slot - j0));
+      static_for<KB>([&](auto kc) { found |= m[decltype(kc)::value] != 0; });
+    };
+    const uint32_t nb = (n_units + KB - 1) / KB;
+    Words w0, w1;
+    load(w0, 0);
+#pragma unroll 1
+    for (uint32_t b = 0; b < nb; b += 2) {
+      load(w1, (b + 1) * KB);
+      test(w0, b * KB);
+      load(w0, (b + 2) * KB);
+      test(w1, (b + 1) * KB);
+    }
+  } else {
+#pragma unroll 1
+    for (uint32_t u = 0; u < n_units; ++u) {
+      const uint64_t off = unit_off(u);
+      static_for<Sh::C>([&](auto cc) {
+        constexpr int J0 = int(decltype(cc)::value) * Sh::NJ;
+        UT w[SlotSpan<UT, BW, J0, Sh::NJ>::NR1];
+        load_slot_words<UT, BW, J0, Sh::NJ>(generic_in(a, off, lane), w);
+        any_test<T, LAYOUT, PATCH, BW, J0, Sh::NJ>(a, w, l, v0, nv, u, vsub, lane, magic, found);
+      });
+    }
+  }
+}
+
+#define FLSGPU_ANY_CASE(B)                                                                  \
+  case B:                                                                                  \
+    if constexpr ((B) <= lanes<UT>::W)                                                      \
+      any_range_uniform<T, LAYOUT, PATCH, (B)>(a, l, v0, nv, n_units, vsub, lane, magic, found); \
+    break;
+
+template <typename T, int LAYOUT, bool PATCH>
+__device__ __forceinline__ void any_range_dispatch(const AlpArgs<T> &a, const AnyLane<T> &l,
+                                                   uint64_t v0, uint32_t nv, uint32_t n_units,
+                                                   int bw, int vsub, int lane, T magic,
+                                                   bool &found) {
+  using UT = typename uint_of<T>::type;
+  if constexpr (lanes<UT>::W == 32) {
+    switch (bw) {
+      FLSGPU_CASES_0_32(FLSGPU_ANY_CASE)
+      default: break;
+    }
+  } else {
+    switch (bw) {
+      FLSGPU_CASES_0_32(FLSGPU_ANY_CASE)
+      FLSGPU_CASES_33_64(FLSGPU_ANY_CASE)
+      default: break;
+    }
+  }
+}
+#undef FLSGPU_ANY_CASE
+
+// One unit of width BW at `off` (a range of mixed widths), in chunks of slots.
+template <typename T, int LAYOUT, bool PATCH, int BW>
+__device__ __forceinline__ void any_unit(const AlpArgs<T> &a, const AnyLane<T> &l, uint64_t v0,
+                                         uint32_t nv, uint32_t u, uint64_t off, int vsub,
+                                         int lane, T magic, bool &found) {
+  using UT = typename uint_of<T>::type;
+  using Sh = AnyShape<UT, BW>;
+  static_for<Sh::C>([&](auto cc) {
+    constexpr int J0 = int(decltype(cc)::value) * Sh::NJ;
+    UT w[SlotSpan<UT, BW, J0, Sh::NJ>::NR1];
+    load_slot_words<UT, BW, J0, Sh::NJ>(generic_in(a, off, lane), w);
+    any_test<T, LAYOUT, PATCH, BW, J0, Sh::NJ>(a, w, l, v0, nv, u, vsub, lane, magic, found);
+  });
+}
+
+#define FLSGPU_ANY_UNIT_CASE(B)                                                              \
+  case B:                                                                                   \
+    if constexpr ((B) <= lanes<UT>::W)                                                       \
+      any_unit<T, LAYOUT, PATCH, (B)>(a, l, v0, nv, u, off, vsub, lane, magic, found);       \
+    break;
+
+template <typename T, int LAYOUT, bool PATCH>
+__device__ __forceinline__ void any_unit_dispatch(const AlpArgs<T> &a, const AnyLane<T> &l,
+                                                  uint64_t v0, uint32_t nv, uint32_t u,
+                                                  uint64_t off, int bw, int vsub, int lane,
+                                                  T magic, bool &found) {
+  using UT = typename uint_of<T>::type;
+  if constexpr (lanes<UT>::W == 32) {
+    switch (bw) {
+      FLSGPU_CASES_0_32(FLSGPU_ANY_UNIT_CASE)
+      default: {  // a width above W: zeros
+        const uint32_t src = any_src<UT>(u, vsub);
+        found |= src < nv && __shfl(l.how, int(src % WAVE), WAVE) == ANY_ALL;
+        break;
+      }
+    }
+  } else {
+    switch (bw) {
+      FLSGPU_CASES_0_32(FLSGPU_ANY_UNIT_CASE)
+      FLSGPU_CASES_33_64(FLSGPU_ANY_UNIT_CASE)
+      default: {
+        const uint32_t src = any_src<UT>(u, vsub);
+        found |= src < nv && __shfl(l.how, int(src % WAVE), WAVE) == ANY_ALL;
+        break;
+      }
+    }
+  }
+}
+#undef FLSGPU_ANY_UNIT_CASE
+
+template <typename T, int LAYOUT, bool PATCH>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(3, 8)))
+void k_alp_any(AlpArgs<T> a, T magic, uint32_t units, Scratch s) {
+  using UT = typename uint_of<T>::type;
+  using L = lanes<UT>;
+  const int t = int(threadIdx.x % WAVE);
+  const int lane = t % L::N, vsub = t / L::N;
+  const uint64_t span = uint64_t(units) * L::VPW;  // vectors per range
+  const uint64_t n_vecs = a.ffor.n_vecs;
+  const uint64_t n_ranges = (n_vecs + span - 1) / span;
+  const uint64_t stride = uint64_t(gridDim.x) * (BLOCK / WAVE);
+  bool found = false;
+  for (uint64_t rr = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE; rr < n_ranges;
+       rr += stride) {
+    const uint64_t v0 = rr * span;
+    const uint32_t nv = uint32_t(n_vecs - v0 < span ? n_vecs - v0 : span);
+    const uint32_t n_units = (nv + L::VPW - 1) / L::VPW;
+    // lane t: vector v0 + t
+    AnyLane<T> l{};
+    int bw = ANY_DEAD_WIDTH;
+    if (uint32_t(t) < nv) {
+      const uint64_t v = v0 + uint64_t(t);
+      bw = a.ffor.bit_widths[v];
+      l.off = a.ffor.vector_offsets[v];
+      l.base = a.ffor.bases[v];
+      AlpConsts<T> c{l.base, a.factor_indices[v], a.fraction_indices[v], 0, T(0)};
+      if constexpr (PATCH) {
+        l.eo = a.exceptions_offsets[v];
+        if constexpr (LAYOUT == ALP_GALP) {  // the lanes' lists follow each other
+          const unsigned oc = a.offsets_counts[v * L::N + (L::N - 1)];
+          l.cnt = (oc & 0x3FFu) + (oc >> 10);
+        } else {
+          l.cnt = a.counts[v];
+        }
+      }
+      c.resolve();
+      l.fact = c.fact;
+      l.frac = c.frac;
+      l.how = alp_target_interval<T>(c, magic, bw, l.lo, l.span);
+    }
+    if constexpr (PATCH) {
+      // the range's exception values: one run when the lists follow each other
+      const uint64_t eo_next = __shfl(l.eo, (t + 1) % WAVE, WAVE);
+      const bool gap = uint32_t(t + 1) < nv && l.eo + l.cnt != eo_next;
+      if (__builtin_amdgcn_ballot_w64(gap) == 0) {
+        const uint64_t e0 = __shfl(l.eo, 0, WAVE);
+        const int last = int(nv) - 1;
+        const uint64_t e1 = __shfl(l.eo, last, WAVE) + __shfl(l.cnt, last, WAVE);
+        for (uint64_t k0 = e0; k0 < e1; k0 += uint64_t(WAVE) * ALP_PF) {
+          T e[ALP_PF];
+#pragma unroll
+          for (int i = 0; i < ALP_PF; ++i) {
+            const uint64_t k = k0 + uint64_t(t + i * WAVE);
+            e[i] = a.exceptions[k < e1 ? k : e0];
+          }
+#pragma unroll
+          for (int i = 0; i < ALP_PF; ++i)
+            found |= k0 + uint64_t(t + i * WAVE) < e1 && e[i] == magic;
+        }
+      } else {
+        for (uint32_t v = 0; v < nv; ++v) {
+          const uint64_t eo = __shfl(l.eo, int(v), WAVE);
+          const unsigned cnt = __shfl(l.cnt, int(v), WAVE);
+          for (unsigned k = unsigned(t); k < cnt; k += WAVE) found |= a.exceptions[eo + k] == magic;
         }
       }
     }
-    found |= m != 0;
+    // Lane 0 holds the range's first vector.
+    const int b0 = __builtin_amdgcn_readfirstlane(bw);
+    if (__builtin_amdgcn_ballot_w64(bw != b0 && bw != ANY_DEAD_WIDTH) == 0) {
+      any_range_dispatch<T, LAYOUT, PATCH>(a, l, v0, nv, n_units, b0, vsub, lane, magic, found);
+    } else {
+#pragma unroll 1
+      for (uint32_t u = 0; u < n_units; ++u) {
+        const uint32_t src = any_src<UT>(u, vsub);
+        const int sl = int(src % WAVE);
+        const bool live = src < nv;
+        const int ubw = __shfl(bw, sl, WAVE);
+        const uint64_t first = uniform_u64(__shfl(l.off, int(u * L::VPW), WAVE));
+        const uint64_t o = __shfl(l.off, sl, WAVE);
+        const uint64_t off = live ? o : first;
+        // The unit's first vector (lane 0's) exists.
+        const int b1 = __builtin_amdgcn_readfirstlane(ubw);
+        if (__builtin_amdgcn_ballot_w64(live && ubw != b1) == 0) {
+          any_unit_dispatch<T, LAYOUT, PATCH>(a, l, v0, nv, u, off, b1, vsub, lane, magic, found);
+        } else {
+          // divergent widths: the runtime-width path, every slot of the lane
+          const int how = live ? __shfl(l.how, sl, WAVE) : int(ANY_NONE);
+          const UT lo = __shfl(l.lo, sl, WAVE), sp = __shfl(l.span, sl, WAVE);
+          AlpConsts<T> c{__shfl(l.base, sl, WAVE), 0, 0, __shfl(l.fact, sl, WAVE),
+                         __shfl(l.frac, sl, WAVE)};
+          const uint64_t eo = __shfl(l.eo, sl, WAVE);
+          const unsigned cnt = __shfl(l.cnt, sl, WAVE);
+          uint64_t m = 0;
+          if (how == ANY_INTERVAL || how == ANY_DECODE) {
+            unpack_runtime<UT>(a.ffor.packed + off + lane, ubw, 0, L::W, [&](int j, UT v) {
+              const bool hit = how == ANY_INTERVAL ? UT(v - lo) <= sp : c(v) == magic;
+              m |= uint64_t(hit) << j;
+            });
+          } else if (how == ANY_ALL) {
+            m = L::W == 64 ? ~uint64_t(0) : ((uint64_t(1) << L::W) - 1);
+          }
+          if constexpr (PATCH) {
+            if (m) m = any_drop_exceptions<T, LAYOUT>(a, v0 + src, lane, 0, L::W, m, eo, cnt);
+          }
+          found |= m != 0;
+        }
+      }
+    }
   }
   block_partial2<RedOp::Or>(found, 0, s.partials);
 }
@@ -789,15 +1214,28 @@ template <typename T, int LAYOUT, bool PATCH>
 int AlpAny<T, LAYOUT, PATCH>::launch(const AlpArgs<T> &a, T magic, uint8_t *d_found,
                                      hipStream_t stream) {
   using UT = typename uint_of<T>::type;
+  using L = lanes<UT>;
   Scratch s;
   int rc = get_scratch(stream, &s);
   if (rc) return rc;
-  const uint64_t n_waves = n_waves_for<UT, alp_any_split<T>()>(a.ffor.n_vecs);
-  // a CU-filling grid whose waves walk several units each (2^26 doubles:
-  // 56 us at 2048 blocks, 61 us with one unit per wave)
-  const unsigned grid = std::min(reduce_grid(n_waves), 2048u);
+  static const unsigned resident = [] {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_alp_any<T, LAYOUT, PATCH>, BLOCK,
+                                                     0) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    return unsigned(per_cu) * unsigned(num_cus());
+  }();
+  // Units per range: the ranges fill the resident waves about once.
+  const uint64_t n_units = n_waves_for<UT, 1>(a.ffor.n_vecs);
+  const uint64_t waves = uint64_t(resident) * (BLOCK / WAVE);
+  const uint32_t units =
+      uint32_t(std::clamp<uint64_t>((n_units + waves - 1) / waves, 1, WAVE / L::VPW));
+  const uint64_t n_ranges = (n_units + units - 1) / units;
+  const uint64_t blocks = (n_ranges + (BLOCK / WAVE) - 1) / (BLOCK / WAVE);
+  const unsigned grid =
+      unsigned(std::min<uint64_t>({std::max<uint64_t>(blocks, 1), resident, 2048}));
   hipLaunchKernelGGL((k_alp_any<T, LAYOUT, PATCH>), dim3(grid), dim3(BLOCK), 0, stream, a, magic,
-                     n_waves, s);
+                     units, s);
   if ((rc = check_hip(hipGetLastError(), "k_alp_any launch"))) return rc;
   return reduce_finish_flag(s, grid, d_found, stream);
 }
