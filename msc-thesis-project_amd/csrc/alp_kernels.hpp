@@ -813,8 +813,9 @@ struct AnyShape {
   static constexpr int C = LANE_DW <= 32 ? 1 : (LANE_DW <= 64 ? 2 : 4);  // chunks per lane
   static constexpr int NJ = W / C;                                       // slots per chunk
   // units per batch: their words within about 32 registers and their slots
-  // within 128 (each slot may need a decode and compare)
-  static constexpr int MAXKB = 128 / W;
+  // within 64 (each slot may need a decode and compare; 128 made the 64-bit
+  // kernel spill registers to scratch)
+  static constexpr int MAXKB = 64 / W;
   static constexpr int FIT = LANE_DW == 0 ? MAXKB : 32 / LANE_DW;
   static constexpr int KB = C > 1 ? 1 : (FIT >= MAXKB ? MAXKB : (FIT >= 2 ? 2 : 1));
 };
