@@ -58,6 +58,10 @@ struct FlsFilterSum {
   // `slots` blocks (the grid is capped at that); allocates nothing.
   static int launch_partials(const FforArgs<UT> &a, UT lo, UT hi, uint64_t *d_result,
                              uint64_t *partials, unsigned slots, hipStream_t stream);
+  // The scan launch alone (its block partials in `partials`, *grid_out
+  // blocks).
+  static int launch_scan(const FforArgs<UT> &a, UT lo, UT hi, uint64_t *partials, unsigned slots,
+                         hipStream_t stream, unsigned *grid_out);
 };
 template <typename UT>
 struct FlsCompute {

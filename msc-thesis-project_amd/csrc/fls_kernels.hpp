@@ -17,6 +17,11 @@
 
 namespace flsgpu_hip {
 
+// The range-scan kernel's predicates: the public FLSGPU_PRED_* plus value ==
+// lo, which serves the "any value == magic" query (its count of matching
+// rows is reduced as a flag).
+constexpr int PRED_EQUAL = 3;
+
 // ---------------------------------------------------------------------------
 // Policies: what happens to each unpacked value.
 // ---------------------------------------------------------------------------
@@ -54,20 +59,6 @@ struct StagePolicy {
   }
 };
 
-template <typename UT>
-struct AnyPolicy {
-  const UT *in;
-  UT base;
-  UT magic;
-  bool found;
-  template <int BW, int J0, int NJ>
-  __device__ __forceinline__ void run() {
-    unpack_slots<UT, BW, J0, NJ>(in, [&](int, UT v) { found |= UT(v + base) == magic; });
-  }
-  __device__ __forceinline__ void run_generic(int bw, int j0, int nj) {
-    unpack_runtime<UT>(in, bw, j0, nj, [&](int, UT v) { found |= UT(v + base) == magic; });
-  }
-};
 
 // Fused filter + sum, runtime-width path (wavefronts whose vectors differ in
 // bit width): exact per value.
@@ -84,6 +75,7 @@ struct FilterSumPolicy {
       bool pass;
       if constexpr (PRED == FLSGPU_PRED_OFFSET_LOWER_HALF) pass = off < half;
       else if constexpr (PRED == FLSGPU_PRED_ALL) pass = true;
+      else if constexpr (PRED == PRED_EQUAL) pass = v == lo;
       else pass = (v >= lo) & (v < hi);
       count += pass;
       sum += pass ? uint64_t(v) : 0;
@@ -105,6 +97,37 @@ __host__ __device__ constexpr uint32_t slot_bit_mask(int word) {
 }
 
 __host__ __device__ constexpr int ilog2(int x) { return x <= 1 ? 0 : 1 + ilog2(x / 2); }
+
+// Calls f(field) for slots [J0, J0 + NJ) of one lane at a width below 32,
+// each field read from the lane's stream as 32-bit words (for 64-bit types
+// the low half of each word first; a field spans at most two of them).
+template <typename UT, int BW, int J0, int NJ, class F>
+__device__ __forceinline__ void span_fields32(const UT *w, F &&f) {
+  static_assert(BW > 0 && BW < 32, "32-bit fields");
+  constexpr int W = lanes<UT>::W;
+  using Sp = SlotSpan<UT, BW, J0, NJ>;
+  constexpr int H0 = Sp::R0 * (W / 32);
+  constexpr int H = Sp::NR * (W / 32);
+  auto hw = [&](auto ic) -> uint32_t {
+    constexpr int i = decltype(ic)::value;
+    if constexpr (i >= H) return 0u;
+    else if constexpr (W == 32) return uint32_t(w[i]);
+    else return uint32_t(uint64_t(w[i / 2]) >> (32 * (i % 2)));
+  };
+  static_for<NJ>([&](auto jc) {
+    constexpr int q = (J0 + int(decltype(jc)::value)) * BW - 32 * H0;
+    constexpr int i = q / 32;
+    constexpr int sh = q % 32;
+    uint32_t x;
+    if constexpr (sh + BW <= 32) {
+      x = hw(std::integral_constant<int, i>{}) >> sh;
+    } else {
+      x = __builtin_amdgcn_alignbit(hw(std::integral_constant<int, i + 1>{}),
+                                    hw(std::integral_constant<int, i>{}), uint32_t(sh));
+    }
+    f(x & ((1u << BW) - 1u));
+  });
+}
 
 // Count of slots [J0, J0 + NJ) of one lane that pass PRED (OFFSET_LOWER_HALF:
 // the FOR offset's top bit is clear; ALL) and the sum of their offsets mod
@@ -223,7 +246,20 @@ __device__ __forceinline__ void span_filter_sum(const UT *w, UT base, UT lo, UT 
   constexpr int W = lanes<UT>::W;
   constexpr UT MAXOFF = BW == W ? UT(~UT(0)) : UT((UT(1) << BW) - 1);
   const bool no_wrap = base <= UT(UT(~UT(0)) - MAXOFF);
-  if constexpr (PRED == FLSGPU_PRED_RANGE) {
+  if constexpr (PRED == PRED_EQUAL) {
+    // base + off == lo (mod 2^W)  <=>  off == lo - base, a count of one
+    // offset (none when that is not a BW-bit offset)
+    const UT want = UT(lo - base);
+    uint32_t cnt = 0;
+    if (want <= MAXOFF) {
+      if constexpr (BW > 0 && BW < 32)
+        span_fields32<UT, BW, J0, NJ>(w, [&](uint32_t off) { cnt += off == uint32_t(want); });
+      else
+        emit_slot_words<UT, BW, J0, NJ>(w, [&](int, UT off) { cnt += off == want; });
+    }
+    count += cnt;
+    sum += uint64_t(cnt) * uint64_t(lo);
+  } else if constexpr (PRED == FLSGPU_PRED_RANGE) {
     using Acc = std::conditional_t<(W == 32 && BW + ilog2(NJ) <= 32), uint32_t, uint64_t>;
     if (no_wrap) {
       // lo <= base + off < hi  <=>  off - olo < span (unsigned)
@@ -384,61 +420,15 @@ __global__ __launch_bounds__(BLOCK) void k_ffor_decompress_staged(FforArgs<UT> a
                           s_stage[threadIdx.x / WAVE]);
 }
 
-// Blocks of the scanning kernels (query any, fused filter + sum): enough to
-// fill every CU (8 blocks of 4 waves on each of 256 CUs); every wave then
-// walks many units, loading the next unit's metadata while it reduces the
-// current one. One unit per wave (a grid of n_waves / 4 blocks) made the
-// narrow widths latency-bound: a wave lived for two dependent memory round
-// trips and a block reduction to decode a few words per lane (u32 width 1
-// filter + sum over 2^28 values: 114 us, now 67 us).
+// Cap on the grid of the range-scan kernel (fused filter + sum, query any):
+// the resident grid is smaller, and the allocation-free entry point's
+// static partials hold this many blocks.
 constexpr unsigned SCAN_GRID = 2048;
 
-// Calls f(unit, bit_width, vector_offset, base) for this wave's units
-// wave0, wave0 + stride, ... below n_waves whose vector exists.
-template <typename UT, int S, class F>
-__device__ __forceinline__ void for_each_unit_prefetched(const FforArgs<UT> &a, uint64_t n_waves,
-                                                         F &&f) {
-  const uint64_t stride = uint64_t(gridDim.x) * (BLOCK / WAVE);
-  uint64_t wave = (uint64_t(blockIdx.x) * BLOCK + threadIdx.x) / WAVE;
-  UnitMap<UT, S> u = map_unit<UT, S>(wave);
-  bool live = wave < n_waves && u.vec < a.n_vecs;
-  int bw = live ? int(a.bit_widths[u.vec]) : 0;
-  uint64_t off = live ? a.vector_offsets[u.vec] : 0;
-  UT base = (live && a.bases) ? a.bases[u.vec] : UT(0);
-  while (wave < n_waves) {
-    const uint64_t wn = wave + stride;
-    const UnitMap<UT, S> un = map_unit<UT, S>(wn);
-    const bool live_n = wn < n_waves && un.vec < a.n_vecs;
-    const int bw_n = live_n ? int(a.bit_widths[un.vec]) : 0;
-    const uint64_t off_n = live_n ? a.vector_offsets[un.vec] : 0;
-    const UT base_n = (live_n && a.bases) ? a.bases[un.vec] : UT(0);
-    if (live) f(u, bw, off, base);
-    wave = wn;
-    u = un;
-    live = live_n;
-    bw = bw_n;
-    off = off_n;
-    base = base_n;
-  }
-}
-
-template <typename UT, int S>
-__global__ __launch_bounds__(BLOCK) void k_ffor_any(FforArgs<UT> a, UT magic, uint64_t n_waves,
-                                                   Scratch s) {
-  bool found = false;
-  for_each_unit_prefetched<UT, S>(a, n_waves, [&](const UnitMap<UT, S> &u, int bw, uint64_t off,
-                                                  UT base) {
-    AnyPolicy<UT> pol{a.packed + off + u.lane, base, magic, false};
-    if (dispatch_bw<UT, S>(pol, bw, u.part)) found |= pol.found;
-    else found |= magic == UT(0);
-  });
-  block_partial2<RedOp::Or>(found, 0, s.partials);
-}
-
-// Fused filter + sum. Every wave takes ranges of `units` consecutive units
-// (units * VPW vectors, at most 64): lane i loads vector i's bit width,
-// offset and base with one coalesced load each, and each unit's threads take
-// theirs by shuffle. When all of a range's vectors have one width, its units
+// Fused filter + sum (and, with PRED_EQUAL, query any). Every wave takes
+// ranges of `units` consecutive units (units * VPW vectors, at most 64): lane
+// i loads vector i's bit width, offset and base with one coalesced load each,
+// and each unit's threads take theirs by shuffle. When all of a range's vectors have one width, its units
 // go in batches of as many as fit about 32 registers of packed words, the
 // next batch's loads issued before the current batch is reduced (two
 // buffers), so a wave always has a batch of loads in flight; at width 1 a
@@ -519,7 +509,9 @@ __device__ __forceinline__ void range_uniform(const FforArgs<UT> &a, const Range
                                               uint64_t &count, uint64_t &sum) {
   constexpr int W = lanes<UT>::W;
   using Sh = ScanShape<UT, BW>;
-  constexpr int KB = Sh::KB;
+  // The equality test works per value on whole words (no bit-plane form):
+  // batches of at most 64 slots keep its registers in bounds.
+  constexpr int KB = PRED == PRED_EQUAL && Sh::KB > 64 / W ? 64 / W : Sh::KB;
   const uint64_t safe_off = uniform_u64(r.off);
   if constexpr (Sh::C == 1) {
     using Words = UT[KB][SlotSpan<UT, BW, 0, W>::NR1];
@@ -582,6 +574,14 @@ __device__ __forceinline__ void range_uniform(const FforArgs<UT> &a, const Range
       unit_filter_sum<UT, PRED, (B)>(a, off, base, live, lane, lo, hi, count, sum); \
     break;
 
+// A vector whose width is above W decodes as zeros (as in the decoder); the
+// fused filter + sum leaves it out, the equality query counts its rows when
+// the needle is zero.
+template <typename UT, int PRED>
+__device__ __forceinline__ void over_wide(bool live, UT lo, uint64_t &count) {
+  if constexpr (PRED == PRED_EQUAL) count += live && lo == UT(0) ? lanes<UT>::W : 0;
+}
+
 template <typename UT, int PRED>
 __device__ __forceinline__ void range_dispatch(const FforArgs<UT> &a, const RangeMeta<UT> &r,
                                                uint32_t n_units, int bw, int lane, UT lo, UT hi,
@@ -589,13 +589,13 @@ __device__ __forceinline__ void range_dispatch(const FforArgs<UT> &a, const Rang
   if constexpr (lanes<UT>::W == 32) {
     switch (bw) {
       FLSGPU_CASES_0_32(FLSGPU_FS_RANGE_CASE)
-      default: break;
+      default: over_wide<UT, PRED>(bw != SCAN_DEAD, lo, count); break;
     }
   } else {
     switch (bw) {
       FLSGPU_CASES_0_32(FLSGPU_FS_RANGE_CASE)
       FLSGPU_CASES_33_64(FLSGPU_FS_RANGE_CASE)
-      default: break;
+      default: over_wide<UT, PRED>(bw != SCAN_DEAD, lo, count); break;
     }
   }
 }
@@ -607,24 +607,26 @@ __device__ __forceinline__ void unit_dispatch(const FforArgs<UT> &a, uint64_t of
   if constexpr (lanes<UT>::W == 32) {
     switch (bw) {
       FLSGPU_CASES_0_32(FLSGPU_FS_UNIT_CASE)
-      default: break;
+      default: over_wide<UT, PRED>(live, lo, count); break;
     }
   } else {
     switch (bw) {
       FLSGPU_CASES_0_32(FLSGPU_FS_UNIT_CASE)
       FLSGPU_CASES_33_64(FLSGPU_FS_UNIT_CASE)
-      default: break;
+      default: over_wide<UT, PRED>(live, lo, count); break;
     }
   }
 }
 #undef FLSGPU_FS_RANGE_CASE
 #undef FLSGPU_FS_UNIT_CASE
 
-// Waves per SIMD the register allocator is held to (the offset-statistics
-// kernels fit without spilling vector registers; the range kernel is left
-// unconstrained).
+// Waves per SIMD the register allocator is held to: the offset-statistics
+// kernels fit three without spilling vector registers, the 64-bit equality
+// test (a 64-bit compare per value) two; the range kernel is left
+// unconstrained.
 template <typename UT, int PRED>
-constexpr int FS_WAVES = PRED == FLSGPU_PRED_RANGE ? 1 : 3;
+constexpr int FS_WAVES =
+    PRED == FLSGPU_PRED_RANGE ? 1 : (PRED == PRED_EQUAL && lanes<UT>::W == 64 ? 2 : 3);
 
 template <typename UT, int S, int PRED>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(FS_WAVES<UT, PRED>, 8)))
@@ -674,6 +676,8 @@ void k_ffor_filter_sum(FforArgs<UT> a, UT lo, UT hi, uint32_t units, Scratch s) 
           pol.run_generic(bw, 0, L::W);
           count += pol.count;
           sum += pol.sum;
+        } else {
+          over_wide<UT, PRED>(live, lo, count);
         }
       }
     }
@@ -752,16 +756,17 @@ int FlsDecompressStaged<UT, S>::launch(const FforArgs<UT> &a, UT *out, hipStream
   return check_hip(hipGetLastError(), "k_ffor_decompress_staged launch");
 }
 
+// "Any value == magic": the range scan counting rows equal to magic, the
+// counts reduced as a flag.
 template <typename UT, int S>
 int FlsAny<UT, S>::launch(const FforArgs<UT> &a, UT magic, uint8_t *d_found, hipStream_t stream) {
   Scratch s;
   int rc = get_scratch(stream, &s);
   if (rc) return rc;
-  const uint64_t n_waves = n_waves_for<UT, S>(a.n_vecs);
-  const unsigned grid = std::min(reduce_grid(n_waves), SCAN_GRID);
-  hipLaunchKernelGGL((k_ffor_any<UT, S>), dim3(grid), dim3(BLOCK), 0, stream, a, magic, n_waves,
-                     s);
-  if ((rc = check_hip(hipGetLastError(), "k_ffor_any launch"))) return rc;
+  unsigned grid = 0;
+  rc = FlsFilterSum<UT, 1, PRED_EQUAL>::launch_scan(a, magic, magic, s.partials, MAX_REDUCE_BLOCKS,
+                                                    stream, &grid);
+  if (rc) return rc;
   return reduce_finish_flag(s, grid, d_found, stream);
 }
 
@@ -775,9 +780,9 @@ int FlsFilterSum<UT, S, PRED>::launch(const FforArgs<UT> &a, UT lo, UT hi, uint6
 }
 
 template <typename UT, int S, int PRED>
-int FlsFilterSum<UT, S, PRED>::launch_partials(const FforArgs<UT> &a, UT lo, UT hi,
-                                               uint64_t *d_result, uint64_t *partials,
-                                               unsigned slots, hipStream_t stream) {
+int FlsFilterSum<UT, S, PRED>::launch_scan(const FforArgs<UT> &a, UT lo, UT hi,
+                                           uint64_t *partials, unsigned slots, hipStream_t stream,
+                                           unsigned *grid_out) {
   static const unsigned resident = [] {
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ffor_filter_sum<UT, S, PRED>, BLOCK,
@@ -798,9 +803,18 @@ int FlsFilterSum<UT, S, PRED>::launch_partials(const FforArgs<UT> &a, UT lo, UT 
   const Scratch s{partials};
   hipLaunchKernelGGL((k_ffor_filter_sum<UT, S, PRED>), dim3(grid), dim3(BLOCK), 0, stream, a, lo,
                      hi, units, s);
-  int rc = check_hip(hipGetLastError(), "k_ffor_filter_sum launch");
+  *grid_out = grid;
+  return check_hip(hipGetLastError(), "k_ffor_filter_sum launch");
+}
+
+template <typename UT, int S, int PRED>
+int FlsFilterSum<UT, S, PRED>::launch_partials(const FforArgs<UT> &a, UT lo, UT hi,
+                                               uint64_t *d_result, uint64_t *partials,
+                                               unsigned slots, hipStream_t stream) {
+  unsigned grid = 0;
+  int rc = launch_scan(a, lo, hi, partials, slots, stream, &grid);
   if (rc) return rc;
-  return reduce_finish_pair(s, grid, d_result, stream);
+  return reduce_finish_pair(Scratch{partials}, grid, d_result, stream);
 }
 
 template <typename UT>
