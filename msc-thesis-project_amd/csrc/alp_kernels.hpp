@@ -957,7 +957,8 @@ __device__ __forceinline__ void any_range_uniform(const AlpArgs<T> &a, const Any
       load(w1, (b + 1) * KB);
       test(w0, b * KB);
       load(w0, (b + 2) * KB);
-      test(w1, (b + 1) * KB);
+      // a batch past the range is loaded but not waited for
+      if (b + 1 < nb) test(w1, (b + 1) * KB);
     }
   } else {
 #pragma unroll 1

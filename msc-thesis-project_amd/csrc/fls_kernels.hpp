@@ -550,7 +550,8 @@ __device__ __forceinline__ void range_uniform(const FforArgs<UT> &a, const Range
       load(w1, (b + 1) * KB);
       reduce(w0, b * KB);
       load(w0, (b + 2) * KB);
-      reduce(w1, (b + 1) * KB);
+      // a batch past the range is loaded but not waited for
+      if (b + 1 < nb) reduce(w1, (b + 1) * KB);
     }
   } else {
 #pragma unroll 1
