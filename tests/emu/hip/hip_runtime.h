@@ -48,6 +48,11 @@ inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz(unsigned(x)); }
 inline int __clzll(long long x) { return x == 0 ? 64 : __builtin_clzll((unsigned long long)x); }
 #define __builtin_nontemporal_store(v, p) (*(p) = (v))
 #define __builtin_amdgcn_readfirstlane(x) (x)
+// v_alignbit_b32: the low 32 bits of the 64-bit {hi, lo} shifted right by s (mod 32)
+inline uint32_t emu_alignbit(uint32_t hi, uint32_t lo, uint32_t s) {
+  return uint32_t(((uint64_t(hi) << 32) | lo) >> (s & 31u));
+}
+#define __builtin_amdgcn_alignbit(hi, lo, s) emu_alignbit(hi, lo, s)
 #define __builtin_amdgcn_fence(...)
 inline int emu_ballot_mode = 0;
 #define __builtin_amdgcn_ballot_w64(p) ((void)(p), uint64_t(emu_ballot_mode))

@@ -7,8 +7,10 @@ AddressSanitizer. Every output buffer has the exact size the library's
 callers give it, so an out-of-bounds access aborts the driver. The encoder's
 arrays must equal the host encoder's byte for byte; the decoders must
 reproduce the values, for every variant shape and both the uniform-width and
-mixed-width unpack paths. Covers the device encoder while its GPU entry
-points are switched off."""
+mixed-width unpack paths. The range-scan kernels' per-lane arithmetic (the
+fused filter + sum's popcount / bit-plane statistics, its range and equality
+predicates, the ALP query's interval of matching offsets) is checked against
+brute force at every bit width."""
 import os
 import shutil
 import struct
@@ -38,7 +40,8 @@ def drivers(tmp_path_factory):
     if cxx is None:
         pytest.skip("no clang++ for the host emulation build")
     out = tmp_path_factory.mktemp("emu")
-    builds = {"encoder_driver": ("encoder_driver", []), "batch_driver": ("batch_driver", [])}
+    builds = {"encoder_driver": ("encoder_driver", []), "batch_driver": ("batch_driver", []),
+              "scan_math_driver": ("scan_math_driver", [])}
     for t in ("float", "double"):
         for g in (0, 1):
             builds[f"alp_decoder_{t}_{g}"] = ("alp_decoder_driver", [f"-DEMU_T={t}", f"-DEMU_GALP={g}"])
@@ -268,3 +271,15 @@ def test_batch_emulated(drivers, tmp_path, ballot):
     assert r.returncode == 0, r.stdout + r.stderr[-3000:]
     runs = [l for l in r.stdout.splitlines() if l.startswith("run ")]
     assert len(runs) == 2 and all(f"0 of {len(specs)} items differ" in l for l in runs), r.stdout
+
+
+def test_scan_arithmetic_brute_force(drivers):
+    """The fused filter + sum's per-lane statistics (every bit width of u32 and
+    u64, every chunk shape, the offset, all-rows, range and equality
+    predicates, bases that do and do not wrap) against a per-value
+    restatement over the same packed words, and the ALP query's interval of
+    matching offsets against the decode of every offset (csrc/fls_kernels.hpp
+    span_filter_sum, csrc/alp_kernels.hpp alp_target_interval)."""
+    r = _run(drivers["scan_math_driver"])
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    assert "mismatches: 0 of" in r.stdout, r.stdout[-2000:]
