@@ -428,15 +428,15 @@ constexpr unsigned SCAN_GRID = 2048;
 // Fused filter + sum (and, with PRED_EQUAL, query any). Every wave takes
 // ranges of `units` consecutive units (units * VPW vectors, at most 64): lane
 // i loads vector i's bit width, offset and base with one coalesced load each,
-// and each unit's threads take theirs by shuffle. When all of a range's vectors have one width, its units
-// go in batches of as many as fit about 32 registers of packed words, the
-// next batch's loads issued before the current batch is reduced (two
-// buffers), so a wave always has a batch of loads in flight; at width 1 a
-// whole range of 64 vectors is one batch of one word per thread. A lane's
-// words beyond that budget (64-bit types from width 17) are read in 2 or 4
-// chunks of slots, one unit at a time. Ranges of mixed widths go unit by
-// unit. The host picks `units` so that the ranges fill the resident waves
-// about once, and caps the grid at what fits on the device at once.
+// and each unit's threads take theirs by shuffle. When all of a range's
+// vectors have one width, its units go in batches of as many as fit about 32
+// registers of packed words (at most 8 units), the next batch's loads issued
+// before the current batch is reduced (two buffers), so a wave always has a
+// batch of loads in flight. A lane's words beyond that budget (64-bit types
+// from width 17) are read in 2 or 4 chunks of slots, one unit at a time.
+// Ranges of mixed widths go unit by unit. The host picks `units` so that the
+// ranges fill the resident waves about once, and caps the grid at what fits
+// on the device at once.
 template <typename UT>
 constexpr int SCAN_RANGE_UNITS = WAVE / lanes<UT>::VPW;  // 64 vectors
 
