@@ -27,9 +27,11 @@
 
 namespace flsgpu_hip {
 
-// Wave-units per task: consecutive units of one column (the parts of one
-// group of vectors, then the next group) run on one wave in a row.
-constexpr int TASK_UNITS = 4;
+// Wave-units per task. One unit per task balances the waves best: on 96
+// columns of 2^12..2^24 values (FFOR u32, G-ALP f64, ALP-RD f64) the launch
+// took 378 us at 1 unit per task, 388 us at 2, 395-398 us at 4 and 422 us
+// at 8; the task list costs 4 bytes per unit.
+constexpr int TASK_UNITS = 1;
 
 // Lane split of the batch's wave-units, one per element width: every
 // encoding of that width shares one unpack body (see batch_unit).
