@@ -6,6 +6,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
+from . import _capi as A
 from . import alp as AL
 
 
@@ -73,9 +74,10 @@ def mixed_batch(seed: int, n_columns: int = 96, min_log2: int = 12, max_log2: in
             if col is None:  # sampling chose ALP-RD for this column
                 specs.append(BatchColumnSpec("alprd_f64", AL.alprd_encode(x), x))
             else:
-                ext = AL.galp_convert(col)  # None if a lane holds more than 63 exceptions
-                specs.append(BatchColumnSpec("alp_ext_f64", ext, x) if ext is not None
-                             else BatchColumnSpec("alp_f64", col, x))
+                try:
+                    specs.append(BatchColumnSpec("alp_ext_f64", AL.galp_convert(col), x))
+                except A.FlsgpuError:  # a lane holds more than 63 exceptions
+                    specs.append(BatchColumnSpec("alp_f64", col, x))
         else:
             x = rng.uniform(0.0, 1.0, n)
             specs.append(BatchColumnSpec("alprd_f64", AL.alprd_encode(x), x))

@@ -68,9 +68,6 @@ def params(x, dtype, rng, mode):
 @pytest.mark.parametrize("mode", ["search", "random"])
 @pytest.mark.parametrize("galp", [False, True])
 def test_alp_device_encode_matches_host(dtype, mode, galp):
-    if galp and mode == "random" and dtype == np.float64:
-        pytest.skip("random parameters leave whole f64 lanes (64 slots) as exceptions, "
-                    "which the 16-bit G-ALP lane header cannot express")
     rng = np.random.default_rng(3)
     x = columns(dtype, 11)
     if galp:  # vectors with a lane of 64 exceptions cannot be expressed in G-ALP
@@ -81,11 +78,20 @@ def test_alp_device_encode_matches_host(dtype, mode, galp):
             v[3] = np.round(v[8], 2)
     e, f = params(x, dtype, rng, mode)
     host = AL.alp_encode_with_params(x, e, f)
-    if galp:
-        host = AL.galp_convert(host)
     xd = torch.from_numpy(x).to(DEV)
-    col, info = ED.alp_encode_device(xd, torch.from_numpy(e).to(DEV), torch.from_numpy(f).to(DEV),
-                                     galp=galp)
+    args = (xd, torch.from_numpy(e).to(DEV), torch.from_numpy(f).to(DEV))
+    if galp:
+        try:
+            host = AL.galp_convert(host)
+        except A.FlsgpuError:
+            # random f64 parameters leave whole lanes (64 slots) as exceptions,
+            # which the 16-bit G-ALP lane header cannot express: the device
+            # encoder must refuse the column as the host converter does
+            assert mode == "random"
+            with pytest.raises(A.FlsgpuError):
+                ED.alp_encode_device(*args, galp=True)
+            return
+    col, info = ED.alp_encode_device(*args, galp=galp)
     nw, ne = info["packed_words"], info["exceptions"]
     assert nw == len(host.packed) and ne == len(host.exceptions)
     assert np.array_equal(t2n(col.packed, nw), host.packed)
