@@ -34,14 +34,20 @@ namespace flsgpu_hip {
 constexpr int TASK_UNITS = 1;
 
 // Lane split of the batch's wave-units, one per element width: every
-// encoding of that width shares one unpack body (see batch_unit).
+// encoding of that width shares one unpack body (see batch_unit). Each unit
+// loads its vectors' metadata and exception-list heads, so finer splits pay
+// that cost more often: on 96 columns of 2^12..2^24 values (FFOR u32, G-ALP
+// f64, ALP-RD f64) 32-bit S = 1 with 64-bit S = 4 took 370-371 us, 32-bit
+// S = 2 376-377 us, 32-bit S = 4 396-398 us, 64-bit S = 8 454-457 us (at
+// five waves per SIMD 444-447 us), 64-bit S = 2 (16 KB stages, two waves
+// per SIMD) 387-392 us.
 template <typename UT>
 constexpr int batch_split() {
-  return sizeof(UT) == 8 ? 4 : 2;
+  return sizeof(UT) == 8 ? 4 : 1;
 }
 
-// LDS stage per wave: VPW * CHUNK elements of the larger shape (64-bit at
-// S = 4: 4 * 16 * 16 * 8 bytes; 32-bit at S = 2: 2 * 32 * 16 * 4 bytes).
+// LDS stage per wave: VPW * CHUNK elements of either shape (64-bit at S = 4:
+// 4 * 16 * 16 * 8 bytes; 32-bit at S = 1: 2 * 32 * 32 * 4 bytes).
 constexpr int BATCH_STAGE_WORDS = 1024;  // 64-bit words
 static_assert(lanes<uint64_t>::VPW * StageShape<uint64_t, batch_split<uint64_t>()>::CHUNK * 8 <=
                   BATCH_STAGE_WORDS * 8 &&
