@@ -352,7 +352,9 @@ typedef struct flsgpu_batch_item {
 
 typedef struct flsgpu_batch_plan flsgpu_batch_plan;
 
-/* Copies the descriptors and a task list to the device (allocates). */
+/* Copies the descriptors and a task list to the device (allocates).
+ * `variant` is ignored: every encoding of one element width runs the batch
+ * kernel's one LDS-staged shape. */
 int flsgpu_batch_plan_create(const flsgpu_batch_item *items, uint32_t n_items, int variant,
                              flsgpu_batch_plan **plan);
 uint64_t flsgpu_batch_plan_tasks(const flsgpu_batch_plan *plan);
