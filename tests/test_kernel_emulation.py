@@ -19,6 +19,7 @@ import subprocess
 import numpy as np
 import pytest
 
+from flsgpu import _capi as A
 from flsgpu import alp as AL
 
 EMU = os.path.join(os.path.dirname(os.path.abspath(__file__)), "emu")
@@ -116,8 +117,6 @@ def same_as_host(arrs, host):
 @pytest.mark.parametrize("galp", [False, True])
 def test_encoder_alp_matches_host(drivers, tmp_path, dtype, mode, galp):
     import test_encode_gpu as TE  # the GPU test's columns and parameters
-    if galp and mode == "random" and dtype == np.float64:
-        pytest.skip("random parameters leave whole f64 lanes as exceptions (no G-ALP header)")
     rng = np.random.default_rng(3)
     x = TE.columns(dtype, 11)
     if galp:
@@ -129,7 +128,15 @@ def test_encoder_alp_matches_host(drivers, tmp_path, dtype, mode, galp):
     e, f = TE.params(x, dtype, rng, mode)
     host = AL.alp_encode_with_params(x, e, f)
     if galp:
-        host = AL.galp_convert(host)
+        try:
+            host = AL.galp_convert(host)
+        except A.FlsgpuError:
+            # random f64 parameters leave whole lanes as exceptions, which the
+            # 16-bit G-ALP header cannot hold: the device encoder reports it
+            assert mode == "random"
+            totals, _ = encode_emulated(drivers, tmp_path, x, e, f, True)
+            assert int(totals[2]) == 1
+            return
     totals, arrs = encode_emulated(drivers, tmp_path, x, e, f, galp)
     assert int(totals[2]) == 0
     same_as_host(arrs, host)
