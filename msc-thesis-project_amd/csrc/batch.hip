@@ -13,9 +13,9 @@
 // another wave, so one launch decodes the batch and can be captured in a
 // hipGraph as it is.
 //
-// Each encoding runs its LDS-staged decoder (decode into the wave's LDS
-// stage, exceptions patched there, 16-byte non-temporal stores) at a fixed
-// lane split: the shapes the per-column tuning picks for most columns.
+// Each encoding runs an LDS-staged decode (decode into the wave's LDS stage,
+// exceptions patched there, 16-byte non-temporal stores) at one lane split
+// per element width (batch_split).
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -98,14 +98,40 @@ __device__ __forceinline__ void pin(const X &x) {
 #endif
 }
 
+// The batch's unpack into the stage: the FOR base added and, for ALP
+// columns (`alp`, wave-uniform), the integer-to-float formula applied before
+// the cell is written, so the stage is not read and written again for it.
+template <typename UT>
+struct BatchStagePolicy {
+  using T = typename float_of<UT>::type;
+  using ST = typename int_of<T>::type;
+  const UT *in;
+  UT *lds;
+  UT base;
+  bool alp;
+  UT fact;
+  T frac;
+  __device__ __forceinline__ UT value(UT v) const {
+    const UT x = UT(v + base);
+    return alp ? __builtin_bit_cast(UT, T(ST(UT(x * fact))) * frac) : x;
+  }
+  template <int BW, int J0, int NJ>
+  __device__ __forceinline__ void run() {
+    unpack_slots<UT, BW, J0, NJ>(in, [&](int j, UT v) { lds[(j - J0) * lanes<UT>::N] = value(v); });
+  }
+  __device__ __forceinline__ void run_generic(int bw, int j0, int nj) {
+    unpack_runtime<UT>(in, bw, j0, nj, [&](int j, UT v) { lds[(j - j0) * lanes<UT>::N] = value(v); });
+  }
+};
+
 // One wave-unit (vector group w / S, lane part w % S) of a column of element
 // width UT, any encoding. The encodings share one unpack body: the packed
-// integers (plus the FOR base) are decoded into the wave's LDS stage, then
-// each thread turns its own cells into the encoding's values (ALP: the
-// integer-to-float formula; ALP-RD: the dictionary's left part joined to the
-// right part), the wave patches the exceptions it loaded before the decode,
-// and the stage is stored with 16-byte non-temporal stores. `dshift`: an
-// ALP-RD column's pre-shifted dictionary in LDS.
+// integers plus the FOR base are decoded into the wave's LDS stage (ALP: with
+// the integer-to-float formula applied on the way), ALP-RD threads then join
+// the dictionary's left part to their own cells' right parts, the wave
+// patches the exceptions it loaded before the decode, and the stage is stored
+// with 16-byte non-temporal stores. `dshift`: an ALP-RD column's pre-shifted
+// dictionary in LDS.
 template <typename UT>
 __device__ __forceinline__ void batch_unit(const BatchColumn &c, uint32_t kind, uint64_t w,
                                            UT *stage, const UT *dshift) {
@@ -196,21 +222,14 @@ __device__ __forceinline__ void batch_unit(const BatchColumn &c, uint32_t kind, 
     }
   }
   UT *row = stage + vv * Sh::CHUNK + u.lane;
-  StagePolicy<UT> pol{in, row, base};
+  BatchStagePolicy<UT> pol{in, row, base, is_alp, is_alp ? alp_factor<T>(fi) : UT(0),
+                           is_alp ? alp_fraction<T>(ei) : T(0)};
   if (!dispatch_bw<UT, S>(pol, bw, u.part)) {
     for (int jj = 0; jj < Sh::NJ; ++jj) row[jj * L::N] = UT(0);
   }
 
-  // 3. the thread's own cells to the encoding's values
-  if (is_alp) {
-    const UT fact = alp_factor<T>(fi);
-    const T frac = alp_fraction<T>(ei);
-#pragma unroll
-    for (int jj = 0; jj < Sh::NJ; ++jj) {
-      const UT x = row[jj * L::N];
-      row[jj * L::N] = __builtin_bit_cast(UT, T(ST(UT(x * fact))) * frac);
-    }
-  } else if (is_rd) {
+  // 3. ALP-RD: the thread's own cells, the dictionary's left part joined
+  if (is_rd) {
     constexpr int Q = 64 / L::N;
     uint64_t lstream[Q];
     left.compose(rd.left_bw, lstream);
